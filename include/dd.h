@@ -1,0 +1,118 @@
+/*
+ * libdd — MI355X (gfx950) distance-driven CT projectors, C ABI.
+ *
+ * Plain pointers and sizes only: device pointers for data, host pointers for
+ * geometry structs, a hipStream_t passed as void*.  Every entry point is
+ * stream-ordered and returns a dd_status; none synchronises the host and none
+ * allocates memory the caller can see.  Scratch space is either passed in
+ * (dd*_workspace_size + the `ws` argument) or, for the reference-named entry
+ * points in deeprecon_hip.h, taken from a per-device pool that the library
+ * grows outside stream capture.
+ *
+ * Reference interface replaced: the raw-pointer C level of deeprecon,
+ *   deeprecon/include/dis2d/{FanArc,FanFlat,Para}DisCUDA2d.h and
+ *   deeprecon/include/dis3d/{ConeFlat,ConeArc,Spiral}DisCUDA3d.h,
+ * called from deeprecon/interface/RadonTorch.cpp.  The geometry structs carry
+ * exactly the fields of the reference's Options tensors
+ * (deeprecon/deeprecon/torch/projector/projector.py:113, :286, :460).
+ */
+#ifndef DD_H_
+#define DD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DD_OK = 0,
+  DD_BAD_ARG = 1,      /* null pointer, non-positive size, unknown op */
+  DD_UNSUPPORTED = 2,  /* op/geometry pair the reference does not define */
+  DD_LAUNCH_FAIL = 3,  /* a kernel launch or HIP call failed */
+  DD_NO_WORKSPACE = 4, /* workspace too small or absent */
+  DD_NO_DEVICE = 5     /* no HIP device visible */
+} dd_status;
+
+/* Scan geometries (the reference's scan_type, projector.py:104,273,452). */
+typedef enum {
+  DD_PARA = 0,     /* 2D parallel beam */
+  DD_FAN_FLAT = 1, /* 2D fan beam, flat detector */
+  DD_FAN_ARC = 2,  /* 2D fan beam, equiangular (arc) detector */
+  DD_CONE_FLAT = 3,
+  DD_CONE_ARC = 4,
+  DD_SPIRAL = 5
+} dd_scan;
+
+/* The six operators of every geometry (RadonTorch.cpp:1376-1418). */
+typedef enum {
+  DD_PROJ = 0,        /* A: image -> sinogram (Proj*) */
+  DD_PROJ_T = 1,      /* A^T, the matched adjoint (ProjTrans*) */
+  DD_BPROJ = 2,       /* FBP-style backprojection (BackProj*) */
+  DD_BPROJ_T = 3,     /* its transpose (BackProjTrans*) */
+  DD_BPROJ_W = 4,     /* distance-weighted backprojection (BackProjWeighted*) */
+  DD_BPROJ_TW = 5     /* its transpose (BackProjTransWeighted*) */
+} dd_op;
+
+/* 2D geometry: the 11-entry Options tensor of projector2d
+ * [W, H, D, dx, dy, det, sid, sdd, shift_x, shift_y, bin_shift]
+ * (projector.py:113).  For DD_FAN_ARC `det` and `bin_shift` are angles in
+ * radians (projector.py:110-112). */
+typedef struct {
+  int width;
+  int height;
+  int num_det;
+  float pix_x;
+  float pix_y;
+  float det;
+  float iso_source;
+  float source_detector;
+  float shift_x;
+  float shift_y;
+  float bin_shift;
+} dd_geom2d;
+
+/* Bytes of scratch device memory dd2d_run needs for this call. */
+size_t dd2d_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
+                           int views);
+
+/* Runs one 2D op on `stream`.
+ *   input  : image [batch, H, W] for DD_PROJ / DD_BPROJ_T / DD_BPROJ_TW,
+ *            otherwise sinogram [batch, views, D]; float32, device.
+ *   output : the other shape; fully overwritten (no pre-zeroing needed).
+ *   angles : [views] float32, device.
+ *   ws     : at least dd2d_workspace_size() bytes of device memory.
+ * Replaces ProjFanArcDisCUDA2d & co. (deeprecon/include/dis2d/FanArcDisCUDA2d.h etc.). */
+dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
+                   float* output, const float* angles, int batch, int views,
+                   void* ws, size_t ws_bytes, void* stream);
+
+/* System matrix with the FP weights as COO entries (row = v*D + bin,
+ * column = row*W + col), built on the device in two passes: count the
+ * entries of every (view, bin) into `counts` [views*D], let the caller turn
+ * them into exclusive offsets, then fill.  Replaces GetSysMatrix*2dKernel
+ * (deeprecon/include/dis2d/FanArcDisCPU2d.h:21-23, FanFlatDisCPU2d.h:21-23,
+ * ParaDisCPU2d.h:16-17), which the reference runs on the CPU only. */
+size_t dd2d_sysmatrix_workspace_size(int scan, const dd_geom2d* g, int views);
+dd_status dd2d_sysmatrix_count(int scan, const dd_geom2d* g,
+                               const float* angles, int views, int64_t* counts,
+                               void* ws, size_t ws_bytes, void* stream);
+dd_status dd2d_sysmatrix_fill(int scan, const dd_geom2d* g,
+                              const float* angles, int views,
+                              const int64_t* offsets, int64_t* rows,
+                              int64_t* cols, float* vals, void* ws,
+                              size_t ws_bytes, void* stream);
+
+/* Instance tile used per launch (1, 2, 4, 8 or 16 instances share one
+ * geometry evaluation); 0 restores the built-in choice.  Process-wide. */
+void dd2d_set_instance_tile(int tile);
+
+/* Library version string. */
+const char* dd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DD_H_ */

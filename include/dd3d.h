@@ -1,0 +1,64 @@
+/*
+ * libdd — 3D distance-driven projectors (cone-flat, cone-arc, spiral), C ABI.
+ * Same conventions as dd.h.
+ *
+ * Reference interface replaced: deeprecon/include/dis3d/ConeFlatDisCUDA3d.h,
+ * ConeArcDisCUDA3d.h and SpiralDisCUDA3d.h (raw-pointer launchers called from
+ * deeprecon/interface/RadonTorch.cpp:572-1267).
+ */
+#ifndef DD3D_H_
+#define DD3D_H_
+
+#include "dd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 3D geometry: the 17-entry Options tensor of projector3d
+ * [W, H, Dz, cols, rows, dx, dy, dz, col, row, sid, sdd, sx, sy, sz,
+ *  col_shift, row_shift] (projector.py:286).  The spiral projector's
+ * 16-entry tensor (projector.py:460) has no sz; pass shift_z = 0.
+ * For DD_CONE_ARC and DD_SPIRAL `col_size` and `col_shift` are radians. */
+typedef struct {
+  int width;
+  int height;
+  int depth;
+  int num_col;
+  int num_row;
+  float pix_x;
+  float pix_y;
+  float pix_z;
+  float col_size;
+  float row_size;
+  float iso_source;
+  float source_detector;
+  float shift_x;
+  float shift_y;
+  float shift_z;
+  float col_shift;
+  float row_shift;
+} dd_geom3d;
+
+size_t dd3d_workspace_size(int scan, int op, const dd_geom3d* g, int batch,
+                           int views);
+
+/* Runs one 3D op on `stream`.
+ *   input  : volume [batch, Dz, H, W] for DD_PROJ / DD_BPROJ_T / DD_BPROJ_TW,
+ *            otherwise sinogram [batch, views, rows, cols]; float32, device.
+ *   output : the other shape, fully overwritten.
+ *   angles : [views] float32, device.
+ *   source_pos : DD_SPIRAL only, [4, views] float32, device: rows are the
+ *            axial source position, radius shift, angle shift and axial
+ *            focal-spot shift (RadonTorch.cpp:1040-1043); NULL otherwise.
+ * Replaces ProjConeFlatDisCUDA3d & co. (deeprecon/include/dis3d/). */
+dd_status dd3d_run(int scan, int op, const dd_geom3d* g, const float* input,
+                   float* output, const float* angles, const float* source_pos,
+                   int batch, int views, void* ws, size_t ws_bytes,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DD3D_H_ */
