@@ -1,0 +1,218 @@
+"""Benchmark: batched autograd FP + adjoint of the fan-arc DD projector.
+
+Workload (one "step"): 16 seeded 512x512 ellipse phantoms (0.7 mm pixels,
+attenuation scaled to 0-0.04 /mm), equiangular fan beam, 720 views over 2*pi,
+736 detector bins of 1.2858 mm at SDD 1085.6 mm, SID 595 mm; y = A x_true with
+Poisson noise at 1e5 incident photons; one step evaluates the gradient of
+||A x - y||^2 at x = x_true + noise through autograd: forward projection A x
+for the whole batch, residual, and the matched adjoint A^T in backward.
+
+* batched path: the step runs on one radontorch.Plan2d (geometry built once,
+  the batch shares every geometry evaluation) and is replayed from a
+  captured HIP graph;
+* plain path: the step issues one radontorch op call per slice and per
+  direction (16 forward + 16 adjoint calls), eagerly.
+
+Prints one JSON line (metric/value/unit/... plus roofline and cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP+BP detector samples/s & %HBM roofline, 512² fan batch16; batched vs plain"
+N, D, V, B = 512, 736, 720, 16
+PIX, DET, SID, SDD = 0.7, 1.2858, 595.0, 1085.6
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 oracle check")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 512 MiB device copy (reference kernel for HBM counters)")
+    return ap.parse_args()
+
+
+def workload(torch, dev):
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    from pridediff_amd.phantom import ellipses2d
+
+    P = projector2d(N, D, PIX, DET, SID, SDD, scan_type="arc").to(dev)
+    ang_np = (np.arange(V, dtype=np.float32) * np.float32(2 * np.pi / V)).astype(np.float32)
+    ang = torch.from_numpy(ang_np).to(dev)
+    x_true = torch.from_numpy(ellipses2d(N, B, seed=2024, scale=0.04))[:, None].to(dev)
+    gen = torch.Generator(device=dev).manual_seed(7)
+    with torch.no_grad():
+        line = P.projection(x_true, ang)
+        counts = torch.poisson(1e5 * torch.exp(-line), generator=gen)
+        y = -torch.log(torch.clamp(counts, min=1.0) / 1e5)
+        x0 = (x_true + 0.002 * torch.randn(x_true.shape, device=dev, generator=gen)).contiguous()
+    return P, ang, ang_np, x_true, x0, y.contiguous()
+
+
+def timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(ang_np, x_np, y_np):
+    """Reference CPU code (oracle/_ref) on 2 slices: FP + ProjTrans, median of 3."""
+    from oracle import oracle as O
+
+    g = O.Geom2D(N, N, D, PIX, PIX, DET / SDD, SID, SDD)
+    mode = "ref" if O.ref_available() else "f32"
+    xs, ys = x_np[:2], y_np[:2]
+    O.op2d("arc", "proj", xs[:1], ang_np, g, mode)  # warm-up
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.op2d("arc", "proj", xs, ang_np, g, mode)
+        O.op2d("arc", "proj_t", ys, ang_np, g, mode)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    hw = os.cpu_count() or 1
+    threads = min(math.ceil(V / 128), hw) if mode == "ref" else 1
+    return {
+        "value": 2 * V * D / t, "unit": "detector samples/s", "cores": threads,
+        "kind": "reference" if mode == "ref" else "port",
+        "sample": (f"2 slices of the 512^2 fan-arc workload (720 views x 736 bins), "
+                   f"Proj + ProjTrans, median of 3 runs ({t:.2f} s each); deeprecon's CPU "
+                   f"thread policy min(ceil(V/128), cores) = {threads} threads on a "
+                   f"{hw}-CPU host"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from pridediff_amd import load_radontorch
+    from pridediff_amd.batched import BatchedProjector2d, GraphStep, data_consistency_grad
+
+    rt = load_radontorch()
+    dev = torch.device("cuda:0")
+    P, ang, ang_np, x_true, x0, y = workload(torch, dev)
+    BP = BatchedProjector2d(P, ang, B)
+
+    # ---- batched path: graph-captured autograd step ----------------------
+    step = GraphStep(lambda x, yy: data_consistency_grad(BP.projection, x, yy),
+                     x0.clone().requires_grad_(True), y)
+    grad_batched = step.replay(x0, y).clone()
+    t_batched = timed(torch, lambda: step.replay(x0, y), args.steps, args.warmup)
+
+    # ---- plain path: one op call per slice and direction, eager ----------
+    def plain_step():
+        grads = []
+        for i in range(B):
+            xi = x0[i:i + 1].detach().clone().requires_grad_(True)
+            grads.append(data_consistency_grad(lambda t: P.projection(t, ang), xi, y[i:i + 1]))
+        return grads
+
+    grad_plain = torch.cat(plain_step())
+    t_plain = timed(torch, plain_step, args.plain_steps, 1)
+
+    # ---- per-kernel timing (HIP events on the launch stream) -------------
+    rt.profile_enable(True)
+    rt.profile_read(0), rt.profile_read(1)
+    for _ in range(5):
+        xg = x0.detach().clone().requires_grad_(True)
+        data_consistency_grad(BP.projection, xg, y)
+    torch.cuda.synchronize()
+    fp_ms, fp_n = rt.profile_read(0)
+    bp_ms, bp_n = rt.profile_read(1)
+    rt.profile_enable(False)
+    fp_ms, bp_ms = fp_ms / max(fp_n, 1), bp_ms / max(bp_n, 1)
+    b_alg = 4.0 * V * (N * N + D) * B  # bytes per operator launch over the batch
+    kernels = {
+        "k_bins (A, image->sino)": {"ms": fp_ms, "GB/s": b_alg / (fp_ms * 1e-3) / 1e9},
+        "k_pixels (A^T, sino->image)": {"ms": bp_ms, "GB/s": b_alg / (bp_ms * 1e-3) / 1e9},
+    }
+    dom_ms = max(fp_ms, bp_ms)
+    dom_name = "k_bins" if fp_ms >= bp_ms else "k_pixels"
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "fanarc_c1_hbm_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get(dom_name)
+
+    if args.copy_probe:
+        src = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+
+    # ---- accuracy --------------------------------------------------------
+    acc = {"batched_vs_plain_grad_rel_l2": float(
+        (grad_batched - grad_plain).double().norm() / grad_plain.double().norm())}
+    x_np = x_true[:, 0].cpu().numpy()
+    y_np = y[:, 0].cpu().numpy()
+    if not args.no_accuracy:
+        from oracle import oracle as O
+        g = O.Geom2D(N, N, D, PIX, PIX, DET / SDD, SID, SDD)
+        with torch.no_grad():
+            fp = BP.projection(x0)[:1, 0].cpu().numpy()
+        want = O.op2d("arc", "proj", x0[:1, 0].cpu().numpy(), ang_np, g, "f64")
+        acc["fp_rel_l2_vs_fp64"] = O.rel_l2(fp, want)
+        acc["fp_max_abs_vs_fp64"] = O.max_abs(fp, want)
+
+    cpu = None if args.no_cpu else cpu_baseline(ang_np, x_np, y_np)
+    samples = B * V * D
+    out = {
+        "metric": METRIC,
+        "value": samples / t_batched,
+        "unit": "detector samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_batched * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipse phantoms, Poisson noise at 1e5 photons)",
+        "config": {"workload": "fan-arc 512x512, 720 views, 736 bins, batch 16, autograd FP + adjoint",
+                   "image": [N, N], "views": V, "bins": D, "batch": B,
+                   "pixel_mm": PIX, "det_mm": DET, "sid_mm": SID, "sdd_mm": SDD},
+        "plain": {"ms_per_step": t_plain * 1e3, "value": samples / t_plain,
+                  "batched_speedup_per_slice": t_plain / t_batched},
+        "roofline": {"bound": "hbm", "achieved": b_alg / (dom_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": b_alg / (dom_ms * 1e-3) / HBM_PEAK, "traffic": traffic,
+                     "kernel": dom_name,
+                     "step_frac": 2 * b_alg / t_batched / HBM_PEAK},
+        "kernels": kernels,
+        "accuracy": acc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

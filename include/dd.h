@@ -104,6 +104,34 @@ dd_status dd2d_sysmatrix_fill(int scan, const dd_geom2d* g,
                               int64_t* cols, float* vals, void* ws,
                               size_t ws_bytes, void* stream);
 
+/* Planned use: the per-view geometry tables (bin edges on the common axis,
+ * FP weights, per-line pixel-edge maps) live in caller memory and are built
+ * once by dd2d_geometry(); dd2d_run_planned() then only moves data.  Used by
+ * the batched path and by graph-captured steps, where the same geometry is
+ * replayed many times.  dd2d_run() is dd2d_geometry() + dd2d_run_planned()
+ * on one workspace. */
+size_t dd2d_geometry_size(int scan, const dd_geom2d* g, int views);
+dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
+                        int views, void* geo, size_t geo_bytes, void* stream);
+size_t dd2d_planned_workspace_size(int scan, int op, const dd_geom2d* g,
+                                   int batch, int views);
+dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
+                           const void* geo, const float* input, float* output,
+                           int batch, int views, void* ws, size_t ws_bytes,
+                           void* stream);
+
+/* Kernel classes timed by the optional profiler. */
+#define DD_KERNEL_IMAGE_TO_SINO 0 /* gathers writing sinograms (FP side) */
+#define DD_KERNEL_SINO_TO_IMAGE 1 /* gathers writing images (adjoint side) */
+#define DD_KERNEL_CLASSES 2
+
+/* Brackets every launch of the gather kernels with HIP events on the launch
+ * stream (never inside stream capture).  dd_profile_read waits for the
+ * recorded events, returns the summed milliseconds and launch count of one
+ * class since the last read, and resets that class. */
+void dd_profile_enable(int on);
+int dd_profile_read(int kernel_class, double* total_ms, long long* launches);
+
 /* Instance tile used per launch (1, 2, 4, 8 or 16 instances share one
  * geometry evaluation); 0 restores the built-in choice.  Process-wide. */
 void dd2d_set_instance_tile(int tile);

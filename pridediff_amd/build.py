@@ -22,11 +22,11 @@ INCLUDE = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["dd2d.hip", "dd3d.hip", "ddbatch.hip"]
+HIP_SOURCES = ["dd2d.hip", "dd3d.hip", "dd_profile.hip", "ddbatch.hip"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=fast",
-    # no packed FP32 math on this target (see the build notes in DESIGN.md)
+    # keep the compiler from emitting packed FP32 (v_pk_*_f32) instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]

@@ -22,6 +22,7 @@
 
 #include "dd.h"
 #include "dd_common.h"
+#include "dd_profile.h"
 
 namespace dd {
 namespace {
@@ -34,7 +35,7 @@ enum WMode { W_FP = 0, W_BP = 1, W_BPW = 2 };
 int g_forced_tile = 0;
 
 struct Geo2 {
-  int W, H, D, V, B, Bp, Lmax, scan;
+  int W, H, D, V, B, Bp, BT, Lmax, scan;
   float dx, dy, det, sid, sdd, shx, shy, bin;
 };
 
@@ -420,7 +421,8 @@ __global__ __launch_bounds__(kThreads) void k_sysmatrix(
 // host side
 // ---------------------------------------------------------------------------
 struct Layout {
-  size_t views, edges, wfp, lines, pint, buf0, buf1, total;
+  size_t views, edges, wfp, lines, pint, geo_total;  // geometry tables
+  size_t buf0, buf1, data_total;                      // data buffers
 };
 
 constexpr size_t kAlign = 256;
@@ -449,6 +451,8 @@ Layout make_layout(const Geo2& g, int op) {
   L.wfp = take(sizeof(float) * g.V * g.D);
   L.lines = take(sizeof(LineGeo) * g.V * g.Lmax);
   L.pint = take(sizeof(float) * g.V * g.Lmax);
+  L.geo_total = off;
+  off = 0;
   if (to_sino(op)) {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
     L.buf1 = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
@@ -456,7 +460,7 @@ Layout make_layout(const Geo2& g, int op) {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * g.D * g.Bp);
     L.buf1 = L.buf0;
   }
-  L.total = off;
+  L.data_total = off;
   return L;
 }
 
@@ -470,6 +474,7 @@ bool fill_geo(Geo2* out, int scan, const dd_geom2d* g, int batch, int views,
   q.D = g->num_det;
   q.V = views;
   q.B = batch;
+  q.BT = tile;
   q.Bp = align_up(batch, tile);
   q.Lmax = std::max(q.W, q.H);
   q.scan = scan;
@@ -485,54 +490,95 @@ bool fill_geo(Geo2* out, int scan, const dd_geom2d* g, int batch, int views,
   return true;
 }
 
+struct GeoPtrs {
+  const ViewGeo* views;
+  const double* edges;
+  const float* wfp;
+  const LineGeo* lines;
+  const float* pint;
+};
+
+GeoPtrs geo_ptrs(const void* geo, const Layout& L) {
+  const char* b = static_cast<const char*>(geo);
+  return {reinterpret_cast<const ViewGeo*>(b + L.views),
+          reinterpret_cast<const double*>(b + L.edges),
+          reinterpret_cast<const float*>(b + L.wfp),
+          reinterpret_cast<const LineGeo*>(b + L.lines),
+          reinterpret_cast<const float*>(b + L.pint)};
+}
+
+void launch_setup(const Geo2& g, const float* angles, void* geo,
+                  const Layout& L, hipStream_t st) {
+  char* b = static_cast<char*>(geo);
+  hipLaunchKernelGGL(k_setup2d, dim3(g.V), dim3(kThreads), 0, st, g, angles,
+                     reinterpret_cast<ViewGeo*>(b + L.views),
+                     reinterpret_cast<double*>(b + L.edges),
+                     reinterpret_cast<float*>(b + L.wfp),
+                     reinterpret_cast<LineGeo*>(b + L.lines),
+                     reinterpret_cast<float*>(b + L.pint));
+}
+
 template <int BT>
 dd_status launch_tile(const Geo2& g, int op, const float* input, float* output,
-                      const float* angles, char* ws, const Layout& L,
+                      const GeoPtrs& G, char* ws, const Layout& L,
                       hipStream_t st) {
-  auto* views = reinterpret_cast<ViewGeo*>(ws + L.views);
-  auto* edges = reinterpret_cast<double*>(ws + L.edges);
-  auto* wfp = reinterpret_cast<float*>(ws + L.wfp);
-  auto* lines = reinterpret_cast<LineGeo*>(ws + L.lines);
-  auto* pint = reinterpret_cast<float*>(ws + L.pint);
   auto* buf0 = reinterpret_cast<float*>(ws + L.buf0);
   auto* buf1 = reinterpret_cast<float*>(ws + L.buf1);
   const int ntile = g.Bp / BT;
-  hipLaunchKernelGGL(k_setup2d, dim3(g.V), dim3(kThreads), 0, st, g, angles,
-                     views, edges, wfp, lines, pint);
   if (to_sino(op)) {
     hipLaunchKernelGGL(k_interleave_image<BT>,
                        dim3((g.W + kThreads - 1) / kThreads, g.H, ntile),
                        dim3(kThreads), 0, st, g, input, buf0, buf1);
     dim3 grid(g.V, (g.D + kThreads - 1) / kThreads, ntile);
+    ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ)
       hipLaunchKernelGGL((k_bins<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                         views, edges, wfp, lines, pint, buf0, buf1, output);
+                         G.views, G.edges, G.wfp, G.lines, G.pint, buf0, buf1, output);
     else if (op == DD_BPROJ_T)
       hipLaunchKernelGGL((k_bins<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                         views, edges, wfp, lines, pint, buf0, buf1, output);
+                         G.views, G.edges, G.wfp, G.lines, G.pint, buf0, buf1, output);
     else
       hipLaunchKernelGGL((k_bins<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                         views, edges, wfp, lines, pint, buf0, buf1, output);
+                         G.views, G.edges, G.wfp, G.lines, G.pint, buf0, buf1, output);
   } else {
     dim3 sgrid((g.D + kThreads - 1) / kThreads, g.V, ntile);
     if (op == DD_PROJ_T)
       hipLaunchKernelGGL((k_interleave_sino<BT, true>), sgrid, dim3(kThreads),
-                         0, st, g, views, wfp, input, buf0);
+                         0, st, g, G.views, G.wfp, input, buf0);
     else
       hipLaunchKernelGGL((k_interleave_sino<BT, false>), sgrid, dim3(kThreads),
-                         0, st, g, views, wfp, input, buf0);
+                         0, st, g, G.views, G.wfp, input, buf0);
     dim3 grid((g.W + kTile - 1) / kTile, (g.H + kTile - 1) / kTile, ntile);
+    ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
     if (op == DD_PROJ_T)
       hipLaunchKernelGGL((k_pixels<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                         views, edges, lines, pint, buf0, output);
+                         G.views, G.edges, G.lines, G.pint, buf0, output);
     else if (op == DD_BPROJ)
       hipLaunchKernelGGL((k_pixels<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                         views, edges, lines, pint, buf0, output);
+                         G.views, G.edges, G.lines, G.pint, buf0, output);
     else
       hipLaunchKernelGGL((k_pixels<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                         views, edges, lines, pint, buf0, output);
+                         G.views, G.edges, G.lines, G.pint, buf0, output);
   }
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+}
+
+dd_status run_tiles(const Geo2& q, int op, const float* input, float* output,
+                    const GeoPtrs& G, char* w, const Layout& L, hipStream_t st) {
+  switch (q.BT) {
+    case 1: return launch_tile<1>(q, op, input, output, G, w, L, st);
+    case 2: return launch_tile<2>(q, op, input, output, G, w, L, st);
+    case 4: return launch_tile<4>(q, op, input, output, G, w, L, st);
+    case 8: return launch_tile<8>(q, op, input, output, G, w, L, st);
+    default: return launch_tile<16>(q, op, input, output, G, w, L, st);
+  }
+}
+
+dd_status check_op(int scan, int op) {
+  if (scan < DD_PARA || scan > DD_FAN_ARC) return DD_BAD_ARG;
+  if (op < DD_PROJ || op > DD_BPROJ_TW) return DD_BAD_ARG;
+  if (scan == DD_PARA && op >= DD_BPROJ_W) return DD_UNSUPPORTED;
+  return DD_OK;
 }
 
 }  // namespace
@@ -552,30 +598,64 @@ size_t dd2d_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
                            int views) {
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return 0;
-  return make_layout(q, op).total;
+  const Layout L = make_layout(q, op);
+  return L.geo_total + L.data_total;
 }
 
 dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
                    float* output, const float* angles, int batch, int views,
                    void* ws, size_t ws_bytes, void* stream) {
-  if (scan < DD_PARA || scan > DD_FAN_ARC) return DD_BAD_ARG;
-  if (op < DD_PROJ || op > DD_BPROJ_TW) return DD_BAD_ARG;
-  if (scan == DD_PARA && op >= DD_BPROJ_W) return DD_UNSUPPORTED;
+  dd_status rc = check_op(scan, op);
+  if (rc != DD_OK) return rc;
   if (!input || !output || !angles) return DD_BAD_ARG;
-  const int tile = pick_tile(batch);
   Geo2 q;
-  if (!fill_geo(&q, scan, g, batch, views, tile)) return DD_BAD_ARG;
+  if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
   const Layout L = make_layout(q, op);
-  if (!ws || ws_bytes < L.total) return DD_NO_WORKSPACE;
+  if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
-  switch (tile) {
-    case 1: return launch_tile<1>(q, op, input, output, angles, w, L, st);
-    case 2: return launch_tile<2>(q, op, input, output, angles, w, L, st);
-    case 4: return launch_tile<4>(q, op, input, output, angles, w, L, st);
-    case 8: return launch_tile<8>(q, op, input, output, angles, w, L, st);
-    default: return launch_tile<16>(q, op, input, output, angles, w, L, st);
-  }
+  launch_setup(q, angles, w, L, st);
+  return run_tiles(q, op, input, output, geo_ptrs(w, L), w + L.geo_total, L, st);
+}
+
+size_t dd2d_geometry_size(int scan, const dd_geom2d* g, int views) {
+  Geo2 q;
+  if (!fill_geo(&q, scan, g, 1, views, 1)) return 0;
+  return make_layout(q, DD_PROJ).geo_total;
+}
+
+dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
+                        int views, void* geo, size_t geo_bytes, void* stream) {
+  dd_status rc = check_op(scan, DD_PROJ);
+  if (rc != DD_OK) return rc;
+  Geo2 q;
+  if (!angles || !fill_geo(&q, scan, g, 1, views, 1)) return DD_BAD_ARG;
+  const Layout L = make_layout(q, DD_PROJ);
+  if (!geo || geo_bytes < L.geo_total) return DD_NO_WORKSPACE;
+  launch_setup(q, angles, geo, L, static_cast<hipStream_t>(stream));
+  return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+}
+
+size_t dd2d_planned_workspace_size(int scan, int op, const dd_geom2d* g,
+                                   int batch, int views) {
+  Geo2 q;
+  if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return 0;
+  return make_layout(q, op).data_total;
+}
+
+dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
+                           const void* geo, const float* input, float* output,
+                           int batch, int views, void* ws, size_t ws_bytes,
+                           void* stream) {
+  dd_status rc = check_op(scan, op);
+  if (rc != DD_OK) return rc;
+  if (!geo || !input || !output) return DD_BAD_ARG;
+  Geo2 q;
+  if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
+  const Layout L = make_layout(q, op);
+  if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
+  return run_tiles(q, op, input, output, geo_ptrs(geo, L),
+                   static_cast<char*>(ws), L, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"
@@ -587,8 +667,7 @@ bool sys_setup(int scan, const dd_geom2d* g, const float* angles, int views,
   if (scan < DD_PARA || scan > DD_FAN_ARC || !angles) return false;
   if (!fill_geo(q, scan, g, 1, views, 1)) return false;
   *L = make_layout(*q, DD_PROJ_T);
-  L->total = L->buf0;  // no data buffer
-  if (!ws || ws_bytes < L->total) return false;
+  if (!ws || ws_bytes < L->geo_total) return false;
   char* w = static_cast<char*>(ws);
   hipLaunchKernelGGL(k_setup2d, dim3(q->V), dim3(kThreads), 0, st, *q, angles,
                      reinterpret_cast<ViewGeo*>(w + L->views),
@@ -605,7 +684,7 @@ extern "C" {
 size_t dd2d_sysmatrix_workspace_size(int scan, const dd_geom2d* g, int views) {
   Geo2 q;
   if (!fill_geo(&q, scan, g, 1, views, 1)) return 0;
-  return make_layout(q, DD_PROJ_T).buf0;
+  return make_layout(q, DD_PROJ_T).geo_total;
 }
 
 dd_status dd2d_sysmatrix_count(int scan, const dd_geom2d* g,

@@ -289,6 +289,85 @@ torch::Tensor op3d(int scan, int op, const torch::Tensor& input,
   return out;
 }
 
+// ---- planned 2D operator (batched path) ---------------------------------------
+// Geometry tables are built once at construction; every call only moves
+// data (interleave + gather), on torch's current stream, with workspaces
+// owned by the plan, so a step replays inside a captured HIP graph without
+// host work.  No counterpart in the reference (which rebuilds its texture
+// and geometry on every call, FanArcDisCUDA2d.cu:758-814).
+class Plan2d {
+ public:
+  Plan2d(int scan, const torch::Tensor& options, const torch::Tensor& angles,
+         int64_t batch)
+      : scan_(scan), batch_(batch) {
+    TORCH_CHECK(scan >= DD_PARA && scan <= DD_FAN_ARC, "Plan2d: 2D scan expected");
+    TORCH_CHECK(angles.is_cuda() && angles.scalar_type() == torch::kFloat32 &&
+                    angles.dim() == 1,
+                "Plan2d: angles must be a 1D float32 device tensor");
+    TORCH_CHECK(batch > 0, "Plan2d: batch must be positive");
+    g_ = decode2d(options, "Plan2d");
+    views_ = angles.size(0);
+    angles_ = angles.contiguous();
+    c10::hip::HIPGuard guard(angles_.device().index());
+    const size_t gb = dd2d_geometry_size(scan, &g_, static_cast<int>(views_));
+    TORCH_CHECK(gb > 0, "Plan2d: invalid geometry");
+    geo_ = workspace(gb, angles_);
+    DD_CHECK(dd2d_geometry(scan, &g_, angles_.data_ptr<float>(),
+                           static_cast<int>(views_), geo_.data_ptr(), gb,
+                           cur_stream(angles_)),
+             "Plan2d geometry");
+    for (int op = DD_PROJ; op <= DD_BPROJ_TW; ++op) {
+      if (scan == DD_PARA && op >= DD_BPROJ_W) break;
+      ws_bytes_[op] = dd2d_planned_workspace_size(scan, op, &g_, static_cast<int>(batch),
+                                                  static_cast<int>(views_));
+      // allocated up front so no allocation happens inside a graph capture
+      ws_[op] = workspace(ws_bytes_[op], angles_);
+    }
+  }
+
+  // Runs op `op` (dd_op) on a [batch, C, ...] input whose B*C equals the
+  // planned batch; returns a new output tensor.
+  torch::Tensor run(int64_t op, const torch::Tensor& input) {
+    TORCH_CHECK(op >= DD_PROJ && op <= DD_BPROJ_TW && ws_bytes_[op] > 0,
+                "Plan2d: op not available for this scan");
+    check_data(input, angles_, "Plan2d");
+    TORCH_CHECK(input.dim() == 4 && input.size(0) * input.size(1) == batch_,
+                "Plan2d: expected a 4D input with B*C == ", batch_);
+    const bool to_sino = (op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW);
+    if (to_sino) {
+      TORCH_CHECK(input.size(2) == g_.height && input.size(3) == g_.width,
+                  "Plan2d: image shape does not match the geometry");
+    } else {
+      TORCH_CHECK(input.size(2) == views_ && input.size(3) == g_.num_det,
+                  "Plan2d: sinogram shape does not match the geometry");
+    }
+    c10::hip::HIPGuard guard(input.device().index());
+    auto out = to_sino
+                   ? torch::empty({input.size(0), input.size(1), views_, g_.num_det},
+                                  input.options())
+                   : torch::empty({input.size(0), input.size(1), g_.height, g_.width},
+                                  input.options());
+    DD_CHECK(dd2d_run_planned(scan_, static_cast<int>(op), &g_, geo_.data_ptr(),
+                              input.data_ptr<float>(), out.data_ptr<float>(),
+                              static_cast<int>(batch_), static_cast<int>(views_),
+                              ws_[op].data_ptr(), ws_bytes_[op], cur_stream(input)),
+             "Plan2d run");
+    return out;
+  }
+
+  int64_t batch() const { return batch_; }
+  int64_t views() const { return views_; }
+
+ private:
+  int scan_;
+  int64_t batch_;
+  int64_t views_ = 0;
+  dd_geom2d g_{};
+  torch::Tensor angles_, geo_;
+  torch::Tensor ws_[6];
+  size_t ws_bytes_[6] = {0, 0, 0, 0, 0, 0};
+};
+
 }  // namespace
 
 // ---- op table (RadonTorch.cpp:1375-1418) -------------------------------------
@@ -394,4 +473,28 @@ PYBIND11_MODULE(radontorch, m) {
   REG(GetSysMatrixFanArc2d);
   REG(GetSysMatrixPara2d);
 #undef REG
+  pybind11::class_<Plan2d>(m, "Plan2d")
+      .def(pybind11::init<int, const torch::Tensor&, const torch::Tensor&, int64_t>(),
+           pybind11::arg("scan"), pybind11::arg("options"), pybind11::arg("angles"),
+           pybind11::arg("batch"))
+      .def("run", &Plan2d::run)
+      .def_property_readonly("batch", &Plan2d::batch)
+      .def_property_readonly("views", &Plan2d::views);
+  m.def("profile_enable", [](bool on) { dd_profile_enable(on ? 1 : 0); });
+  m.def("profile_read", [](int cls) {
+    double ms = 0.0;
+    long long n = 0;
+    dd_profile_read(cls, &ms, &n);
+    return pybind11::make_tuple(ms, n);
+  });
+  m.def("set_instance_tile", [](int t) { dd2d_set_instance_tile(t); });
+  m.attr("PROJ") = static_cast<int>(DD_PROJ);
+  m.attr("PROJ_T") = static_cast<int>(DD_PROJ_T);
+  m.attr("BPROJ") = static_cast<int>(DD_BPROJ);
+  m.attr("BPROJ_T") = static_cast<int>(DD_BPROJ_T);
+  m.attr("BPROJ_W") = static_cast<int>(DD_BPROJ_W);
+  m.attr("BPROJ_TW") = static_cast<int>(DD_BPROJ_TW);
+  m.attr("SCAN") = pybind11::dict(pybind11::arg("para") = static_cast<int>(DD_PARA),
+                                  pybind11::arg("flat") = static_cast<int>(DD_FAN_FLAT),
+                                  pybind11::arg("arc") = static_cast<int>(DD_FAN_ARC));
 }
