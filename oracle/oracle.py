@@ -138,7 +138,7 @@ def op2d(geom: str, op: str, x: np.ndarray, angles: np.ndarray, g: Geom2D,
         ang = np.ascontiguousarray(angles, dtype=np.float64)
         out = np.zeros(out_shape, dtype=np.float64)
         img, sino = (xin, out) if to_sino else (out, xin)
-        params = [float(v) for v in (g.dx, g.dy, g.det, g.sid, g.sdd, g.shx, g.shy, g.bin)]
+        params = g.f32()  # the op receives float32 Options
         rc = lib.ddo_op2d_f64(gi, oi, img.ctypes.data_as(_F64P), sino.ctypes.data_as(_F64P),
                               ang.ctypes.data_as(_F64P), batch, g.W, g.H, views, g.D, *params)
     else:
@@ -182,6 +182,74 @@ def _bind3d(lib):
     i, d = ctypes.c_int, ctypes.c_double
     lib.ddo_op3d_f64.argtypes = [i, i, _F64P, _F64P, _F64P, _F64P] + [i] * 7 + [d] * 12
     lib.ddo_op3d_f64.restype = i
+
+
+@dataclass(frozen=True)
+class Geom3D:
+    """The reference's 3D Options: [W, H, Dz, cols, rows, dx, dy, dz, col, row,
+    sid, sdd, sx, sy, sz, col_shift, row_shift] (projector.py:286); for the
+    spiral the 16-entry form without sz (projector.py:460).  Arc / spiral
+    column size and shift are in radians."""
+    W: int
+    H: int
+    Dz: int
+    C: int
+    R: int
+    dx: float
+    dy: float
+    dz: float
+    col: float
+    row: float
+    sid: float
+    sdd: float
+    shx: float = 0.0
+    shy: float = 0.0
+    shz: float = 0.0
+    csh: float = 0.0
+    rsh: float = 0.0
+
+    def options(self, spiral: bool = False):
+        o = [self.W, self.H, self.Dz, self.C, self.R, self.dx, self.dy, self.dz, self.col,
+             self.row, self.sid, self.sdd, self.shx, self.shy]
+        return o + ([self.csh, self.rsh] if spiral else [self.shz, self.csh, self.rsh])
+
+
+SCANS3D = {"cone_flat": 3, "cone_arc": 4, "spiral": 5}
+
+
+def op3d(scan: str, op: str, x: np.ndarray, angles: np.ndarray, g: Geom3D,
+         spos: np.ndarray | None = None) -> np.ndarray:
+    """3D op in FP64 (restatement of the reference's CUDA kernels).
+
+    ``x`` is a volume ``[B, Dz, H, W]`` for image->sinogram ops, else a
+    sinogram ``[B, V, rows, cols]``; ``spos`` is ``[4, V]`` for the spiral.
+    Parameters are taken at float32 precision, as the op receives them.
+    """
+    lib = _load_oracle()
+    views = int(np.asarray(angles).shape[0])
+    batch = int(x.shape[0])
+    img_shape, sino_shape = (batch, g.Dz, g.H, g.W), (batch, views, g.R, g.C)
+    to_sino = op in TO_SINO
+    if tuple(x.shape) != (img_shape if to_sino else sino_shape):
+        raise ValueError(f"input shape {x.shape} does not match the geometry")
+    xin = np.ascontiguousarray(x, dtype=np.float64)
+    ang = np.ascontiguousarray(np.asarray(angles, dtype=np.float32), dtype=np.float64)
+    out = np.zeros(sino_shape if to_sino else img_shape, dtype=np.float64)
+    img, sino = (xin, out) if to_sino else (out, xin)
+    sp = None
+    if scan == "spiral":
+        if spos is None:
+            raise ValueError("spiral needs source_pos")
+        sp = np.ascontiguousarray(np.asarray(spos, dtype=np.float32), dtype=np.float64)
+    params = [float(np.float32(v)) for v in (g.dx, g.dy, g.dz, g.col, g.row, g.sid, g.sdd,
+                                             g.shx, g.shy, g.shz, g.csh, g.rsh)]
+    rc = lib.ddo_op3d_f64(SCANS3D[scan], OPS[op], img.ctypes.data_as(_F64P), sino.ctypes.data_as(_F64P),
+                          ang.ctypes.data_as(_F64P),
+                          sp.ctypes.data_as(_F64P) if sp is not None else None,
+                          batch, g.W, g.H, g.Dz, views, g.C, g.R, *params)
+    if rc != 0:
+        raise ValueError(f"unsupported 3D call {scan}/{op}")
+    return out
 
 
 def rel_l2(a, b) -> float:

@@ -23,7 +23,8 @@ def _axis(view32: np.float32):
 def matrices(geom: str, g, angles: np.ndarray):
     """Returns dense ``(M_fp, M_bp, M_bpw)`` of shape ``[V*D, H*W]``."""
     V, D, W, H = len(angles), g.D, g.W, g.H
-    dx, dy, det, sid, sdd = g.dx, g.dy, g.det, g.sid, g.sdd
+    # geometry at float32 precision, as the Options tensor carries it
+    dx, dy, det, sid, sdd, shx, shy, bin_ = g.f32()
     M_fp = np.zeros((V * D, H * W))
     M_bp = np.zeros_like(M_fp)
     M_bpw = np.zeros_like(M_fp)
@@ -35,15 +36,15 @@ def matrices(geom: str, g, angles: np.ndarray):
         j = np.arange(D + 1, dtype=np.float64)
         # detector edges mapped onto the common axis, in detector order
         if geom == "arc":
-            gam = (j - D / 2.0) * det + g.bin + th
+            gam = (j - D / 2.0) * det + bin_ + th
             e = sx + sy * np.tan(gam) if xaxis else sy + sx / np.tan(gam)
         elif geom == "flat":
-            vdet, vbin = sid / sdd * det, sid / sdd * g.bin
+            vdet, vbin = sid / sdd * det, sid / sdd * bin_
             u = (j - D / 2.0) * vdet + vbin
             px, py = u * c, u * s
             e = (sx * py - sy * px) / (py - sy) if xaxis else (sy * px - sx * py) / (px - sx)
         else:
-            u = (j - D / 2.0) * det + g.bin
+            u = (j - D / 2.0) * det + bin_
             e = u / c if xaxis else u / s
         lo_b, hi_b = np.minimum(e[:-1], e[1:]), np.maximum(e[:-1], e[1:])
         width = hi_b - lo_b
@@ -56,8 +57,8 @@ def matrices(geom: str, g, angles: np.ndarray):
             w_fp = dx / (width * (abs(sx) / np.sqrt((mid - sy) ** 2 + sx ** 2)))
         for r in range(H):
             for col in range(W):
-                xc = (col - W / 2.0 + 0.5) * dx + g.shx
-                yc = (H / 2.0 - r - 0.5) * dy + g.shy
+                xc = (col - W / 2.0 + 0.5) * dx + shx
+                yc = (H / 2.0 - r - 0.5) * dy + shy
                 if xaxis:
                     xe = np.array([xc - dx / 2, xc + dx / 2])
                     if geom == "para":
