@@ -153,12 +153,18 @@ void run3d(const P3& g, int op, const double* angles, const double* in,
             }
           }
           const int kk = flip ? C - 1 - k : k;
-          // z merge: slices [zl0 + s pz, +pz] against rows [zrow(k,r), zrow(k,r+1)]
-          for (int sl = 0; sl < g.Dz; ++sl) {
+          // z merge: slices [zl0 + s pz, +pz] against rows [zrow(k,r), zrow(k,r+1)],
+          // both increasing (the reference's nested walk, ConeFlatDisCUDA3d.cu:131-166)
+          int sl = 0, r = 0;
+          while (sl < g.Dz && r < R) {
             const double za = zl0 + sl * pz, zb = za + pz;
-            for (int r = 0; r < R; ++r) {
-              const double oz = std::fmin(zb, zrow(k, r + 1)) - std::fmax(za, zrow(k, r));
-              if (oz <= 0) continue;
+            const double ra = zrow(k, r), rb = zrow(k, r + 1);
+            const double oz = std::fmin(zb, rb) - std::fmax(za, ra);
+            const int s_here = sl, r_here = r;
+            if (zb < rb) ++sl; else ++r;
+            if (oz <= 0) continue;
+            {
+              const int sl = s_here, r = r_here;
               const double wfp_v = (op == kProj || op == kProjTrans) ? wfp(k, r) * ox * oz : 0.0;
               const double wbp_v = kbp * (ox / p) * (oz / pz) * fbp;
               const double w = (op == kProj || op == kProjTrans) ? wfp_v : wbp_v;
