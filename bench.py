@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 oracle check")
+    ap.add_argument("--no-c2", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -75,6 +76,58 @@ def timed(torch, fn, steps, warmup):
         fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def uneven_c2(torch, dev, steps, warmup, check):
+    """64 fan-arc instances, N in {256,384,512}, V in [96,720], D = ceil(1.44 N):
+    one persistent launch per op inside one HIP graph vs 64 plain calls."""
+    from pridediff_amd.autotune import tune_uneven
+    from pridediff_amd.batched import GraphStep, UnevenBatch2d, data_consistency_grad
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    from pridediff_amd.phantom import ellipses2d
+
+    rng = np.random.default_rng(2025)
+    projs, angs, angs_np, imgs = [], [], [], []
+    for i in range(64):
+        n = int(rng.choice([256, 384, 512]))
+        v = int(rng.integers(96, 721))
+        d = int(math.ceil(1.44 * n))
+        projs.append(projector2d(n, d, PIX, DET, SID, SDD, scan_type="arc").to(dev))
+        a = (np.arange(v, dtype=np.float32) * np.float32(2 * np.pi / v)).astype(np.float32)
+        angs_np.append(a)
+        angs.append(torch.from_numpy(a).to(dev))
+        imgs.append(torch.from_numpy(ellipses2d(n, 1, seed=3000 + i, scale=0.04))[0].to(dev))
+    ub = UnevenBatch2d(projs, angs)
+    tune = {}
+    tune_uneven(ub, report=tune)
+    x = ub.pack_images(imgs)
+    with torch.no_grad():
+        y = (ub.projection(x) * 1.001).contiguous()
+    step = GraphStep(lambda a, b: data_consistency_grad(ub.projection, a, b),
+                     x.clone().requires_grad_(True), y)
+    t_b = timed(torch, lambda: step.replay(x, y), steps, warmup)
+    ys = ub.unpack_sinos(y)
+
+    def plain():
+        for P, a, im, yy in zip(projs, angs, imgs, ys):
+            xi = im[None, None].detach().clone().requires_grad_(True)
+            data_consistency_grad(lambda t: P.projection(t, a), xi, yy[None, None])
+    t_p = timed(torch, plain, 2, 1)
+    shapes = ub.shapes
+    b_alg = sum(8.0 * v * (h * w + d) for h, w, v, d in shapes)
+    out = {"instances": 64, "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
+           "speedup": t_p / t_b, "roofline_frac": b_alg / t_b / HBM_PEAK,
+           "algorithmic_GB": b_alg / 1e9, "grid": tune.get("grid")}
+    if check:
+        from oracle import oracle as O
+        j = min(range(64), key=lambda i: shapes[i][0] * shapes[i][2])
+        h, w, v, d = shapes[j]
+        g = O.Geom2D(w, h, d, PIX, PIX, DET / SDD, SID, SDD)
+        with torch.no_grad():
+            got = ub.unpack_sinos(ub.projection(x))[j].cpu().numpy()[None]
+        want = O.op2d("arc", "proj", imgs[j].cpu().numpy()[None], angs_np[j], g, "f64")
+        out["smallest_instance_fp_rel_l2_vs_fp64"] = O.rel_l2(got, want)
+    return out
 
 
 def cpu_baseline(ang_np, x_np, y_np):
@@ -113,12 +166,14 @@ def main():
     import torch
 
     from pridediff_amd import load_radontorch
-    from pridediff_amd.batched import BatchedProjector2d, GraphStep, data_consistency_grad
+    from pridediff_amd.autotune import tune_uniform
+    from pridediff_amd.batched import GraphStep, data_consistency_grad
 
     rt = load_radontorch()
     dev = torch.device("cuda:0")
     P, ang, ang_np, x_true, x0, y = workload(torch, dev)
-    BP = BatchedProjector2d(P, ang, B)
+    tune = {}
+    BP = tune_uniform(P, ang, B, report=tune)
 
     # ---- batched path: graph-captured autograd step ----------------------
     step = GraphStep(lambda x, yy: data_consistency_grad(BP.projection, x, yy),
@@ -182,6 +237,8 @@ def main():
         acc["fp_rel_l2_vs_fp64"] = O.rel_l2(fp, want)
         acc["fp_max_abs_vs_fp64"] = O.max_abs(fp, want)
 
+    c2 = None if args.no_c2 else uneven_c2(torch, dev, max(3, args.steps // 2), 2,
+                                           not args.no_accuracy)
     cpu = None if args.no_cpu else cpu_baseline(ang_np, x_np, y_np)
     samples = B * V * D
     out = {
@@ -202,12 +259,14 @@ def main():
                    "pixel_mm": PIX, "det_mm": DET, "sid_mm": SID, "sdd_mm": SDD},
         "plain": {"ms_per_step": t_plain * 1e3, "value": samples / t_plain,
                   "batched_speedup_per_slice": t_plain / t_batched},
+        "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": b_alg / (dom_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": b_alg / (dom_ms * 1e-3) / HBM_PEAK, "traffic": traffic,
                      "kernel": dom_name,
                      "step_frac": 2 * b_alg / t_batched / HBM_PEAK},
         "kernels": kernels,
+        "uneven_c2": c2,
         "accuracy": acc,
         "cpu_baseline": cpu,
     }

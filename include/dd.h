@@ -113,12 +113,59 @@ dd_status dd2d_sysmatrix_fill(int scan, const dd_geom2d* g,
 size_t dd2d_geometry_size(int scan, const dd_geom2d* g, int views);
 dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
                         int views, void* geo, size_t geo_bytes, void* stream);
+/* Kernel variants of the sinogram -> image gathers (all exact and equal):
+ * each pixel finds its bins on its own, or a 16x16 / 8x8 pixel tile stages
+ * its lines' bin-edge positions per view in LDS. */
+#define DD_ADJ_PER_PIXEL 0
+#define DD_ADJ_TILE16 1
+#define DD_ADJ_TILE8 2
+
+/* `tile` = instances sharing one geometry evaluation (1, 2, 4, 8 or 16;
+ * 0 = automatic); it must be the same in both calls.  `variant` selects the
+ * sinogram -> image kernel (DD_ADJ_*; -1 = process default). */
 size_t dd2d_planned_workspace_size(int scan, int op, const dd_geom2d* g,
-                                   int batch, int views);
+                                   int batch, int views, int tile);
 dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
                            const void* geo, const float* input, float* output,
-                           int batch, int views, void* ws, size_t ws_bytes,
+                           int batch, int views, int tile, int variant,
+                           void* ws, size_t ws_bytes, void* stream);
+
+/* Batched projection of many instances of uneven geometry.
+ *
+ * Each item describes one instance (or `batch` instances sharing a
+ * geometry) and where its image and sinogram live inside one packed image
+ * buffer and one packed sinogram buffer (offsets in floats; image
+ * [batch, H, W], sinogram [batch, views, D]).  dd2d_batch_create is host
+ * only; dd2d_batch_setup builds every instance's geometry in the device
+ * memory it is given (dd2d_batch_device_bytes) and waits for it once;
+ * dd2d_batch_run then runs one op for all instances as two persistent
+ * launches (layout, gather), capturable into a HIP graph.  The result for
+ * each instance equals dd2d_run on that instance bit for bit.  No
+ * counterpart in the reference, which launches once per instance and op
+ * (deeprecon/cuda/dis2d/FanArcDisCUDA2d.cu:758-1016). */
+typedef struct {
+  int scan;               /* DD_PARA, DD_FAN_FLAT or DD_FAN_ARC */
+  dd_geom2d geom;
+  int views;
+  int batch;              /* instances sharing this geometry */
+  const float* angles;    /* [views], device */
+  long long image_offset; /* floats into the packed image buffer */
+  long long sino_offset;  /* floats into the packed sinogram buffer */
+} dd2d_batch_item;
+
+typedef struct dd2d_batch dd2d_batch;
+
+dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n);
+size_t dd2d_batch_device_bytes(const dd2d_batch* b);
+/* Persistent grid size in workgroups (0 = 8 per compute unit). */
+void dd2d_batch_set_grid(dd2d_batch* b, int workgroups);
+/* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL or DD_ADJ_TILE16. */
+void dd2d_batch_set_variant(dd2d_batch* b, int variant);
+dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes,
                            void* stream);
+dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input,
+                         float* output, void* stream);
+void dd2d_batch_destroy(dd2d_batch* b);
 
 /* Kernel classes timed by the optional profiler. */
 #define DD_KERNEL_IMAGE_TO_SINO 0 /* gathers writing sinograms (FP side) */
@@ -131,6 +178,9 @@ dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
  * class since the last read, and resets that class. */
 void dd_profile_enable(int on);
 int dd_profile_read(int kernel_class, double* total_ms, long long* launches);
+
+/* Default sinogram -> image kernel variant of dd2d_run (DD_ADJ_*). */
+void dd2d_set_adjoint_variant(int variant);
 
 /* Instance tile used per launch (1, 2, 4, 8 or 16 instances share one
  * geometry evaluation); 0 restores the built-in choice.  Process-wide. */

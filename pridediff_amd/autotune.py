@@ -1,0 +1,150 @@
+"""Autotuner for the batched projector paths.
+
+Sweeps the launch parameters of a planned operator on the problem it will
+run — the instance tile (how many instances share one geometry evaluation,
+1-16) and the sinogram -> image kernel variant (per pixel, 16x16 or 8x8
+LDS-staged pixel tiles) of a uniform batch, the persistent grid size and
+kernel variant of an uneven batch —
+times one forward projection plus one adjoint per candidate with HIP events,
+keeps the fastest, and remembers it in a JSON cache keyed by the device and
+the geometry, so later runs skip the sweep.
+
+Cache file: ``$PRIDEDIFF_AMD_TUNE_CACHE`` or
+``~/.cache/pridediff_amd/autotune.json``.
+"""
+from __future__ import annotations
+
+import json
+import os
+import statistics
+
+import torch
+
+from pridediff_amd.batched import BatchedProjector2d, UnevenBatch2d, rt
+
+CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
+
+
+KEY_VERSION = 2  # bumped whenever the kernels or the sweep change
+
+
+def _device_key() -> str:
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    arch = getattr(p, "gcnArchName", "") or p.name
+    return f"v{KEY_VERSION}|{arch}|{p.multi_processor_count}cu"
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        tmp = CACHE + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, CACHE)
+    except OSError:
+        pass  # a read-only home only costs a re-sweep next time
+
+
+def time_ms(fn, reps: int = 5, warmup: int = 2) -> float:
+    """Median milliseconds of ``fn()`` on the current stream (HIP events)."""
+    for _ in range(warmup):
+        fn()
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    return statistics.median(times)
+
+
+def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8, 16),
+                 variants=None, use_cache: bool = True,
+                 report: dict | None = None) -> BatchedProjector2d:
+    """Fastest ``BatchedProjector2d`` (tile, kernel variant) for this
+    projector, angle set and batch."""
+    if variants is None:
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8)
+    opts = [round(float(v), 6) for v in projector.host_options().tolist()]
+    key = f"{_device_key()}|uniform|{projector.scan_type}|{opts}|V={angles.numel()}|B={batch}"
+    table = _load() if use_cache else {}
+    if key in table:
+        if report is not None:
+            report.update(table[key])
+        return BatchedProjector2d(projector, angles, batch, tile=table[key]["tile"],
+                                  variant=table[key]["variant"])
+    h, w = projector.image_size[1], projector.image_size[0]
+    x = torch.rand(batch, 1, h, w, device=angles.device)
+    y = torch.rand(batch, 1, angles.numel(), projector.num_det, device=angles.device)
+    results = {}
+    for t in tiles:
+        if t > 1 and t // 2 >= batch:
+            continue
+        for var in variants:
+            bp = BatchedProjector2d(projector, angles, batch, tile=t, variant=var)
+            with torch.no_grad():
+                results[(t, var)] = time_ms(lambda: (bp.plan.run(rt.PROJ, x),
+                                                     bp.plan.run(rt.PROJ_T, y)))
+            del bp
+    best = min(results, key=results.get)
+    entry = {"tile": best[0], "variant": best[1], "ms": results[best],
+             "swept": {f"tile{k[0]}/var{k[1]}": v for k, v in results.items()}}
+    if use_cache:
+        table = _load()
+        table[key] = entry
+        _store(table)
+    if report is not None:
+        report.update(entry)
+    return BatchedProjector2d(projector, angles, batch, tile=best[0], variant=best[1])
+
+
+def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(1, 2, 4, 8, 16), variants=None,
+                use_cache: bool = True, report: dict | None = None) -> UnevenBatch2d:
+    """Sets the fastest persistent grid size and kernel variant on ``batch``
+    (in place)."""
+    if variants is None:
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16)
+    cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    sig = sorted(batch.shapes)
+    key = f"{_device_key()}|uneven|{batch.scan_type}|{sig}"
+    table = _load() if use_cache else {}
+    if key in table:
+        batch.plan.set_grid(table[key]["grid"])
+        batch.plan.set_variant(table[key]["variant"])
+        if report is not None:
+            report.update(table[key])
+        return batch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    x = torch.rand(batch.plan.image_size, device=dev)
+    y = torch.rand(batch.plan.sino_size, device=dev)
+    results = {}
+    for gpc in grids_per_cu:
+        for var in variants:
+            batch.plan.set_grid(gpc * cus)
+            batch.plan.set_variant(var)
+            with torch.no_grad():
+                results[(gpc * cus, var)] = time_ms(lambda: (batch.plan.run(rt.PROJ, x),
+                                                             batch.plan.run(rt.PROJ_T, y)))
+    best = min(results, key=results.get)
+    batch.plan.set_grid(best[0])
+    batch.plan.set_variant(best[1])
+    entry = {"grid": best[0], "variant": best[1], "ms": results[best],
+             "swept": {f"grid{k[0]}/var{k[1]}": v for k, v in results.items()}}
+    if use_cache:
+        table = _load()
+        table[key] = entry
+        _store(table)
+    if report is not None:
+        report.update(entry)
+    return batch

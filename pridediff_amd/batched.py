@@ -42,12 +42,16 @@ class BatchedProjector2d(torch.nn.Module):
               "backprojection": ("BPROJ", "BPROJ_T"), "backprojection_t": ("BPROJ_T", "BPROJ"),
               "backprojection_w": ("BPROJ_W", "BPROJ_TW")}
 
-    def __init__(self, projector, angles: torch.Tensor, batch: int):
+    def __init__(self, projector, angles: torch.Tensor, batch: int, tile: int = 0,
+                 variant: int = -1):
         super().__init__()
         self.scan_type = projector.scan_type
         self.plan = rt.Plan2d(rt.SCAN[projector.scan_type], projector.host_options(),
-                              angles.contiguous().to(torch.float32), int(batch))
+                              angles.contiguous().to(torch.float32), int(batch), int(tile),
+                              int(variant))
         self.batch = int(batch)
+        self.tile = self.plan.tile
+        self.variant = self.plan.variant
 
     def _op(self, name, x):
         a, b = self._PAIRS[name]
@@ -72,6 +76,50 @@ class BatchedProjector2d(torch.nn.Module):
 
     def forward(self, image):
         return self.projection(image)
+
+
+class UnevenBatch2d(torch.nn.Module):
+    """Instances of different image size, detector and view count, all run
+    by one persistent launch per op (``radontorch.PlanBatch2d``).
+
+    Images and sinograms travel packed into flat buffers (``pack_images`` /
+    ``unpack_sinos`` and friends); the ops are differentiable.
+    """
+
+    def __init__(self, projectors, angles, grid: int = 0):
+        super().__init__()
+        scans = {p.scan_type for p in projectors}
+        assert len(scans) == 1, "all instances of a batch must share the scan type"
+        self.scan_type = scans.pop()
+        self.plan = rt.PlanBatch2d(rt.SCAN[self.scan_type], [p.host_options() for p in projectors],
+                                   [a.contiguous().to(torch.float32) for a in angles], int(grid))
+        self.shapes = [tuple(s) for s in self.plan.shapes]   # (H, W, V, D)
+        self.image_offsets = list(self.plan.image_offsets)
+        self.sino_offsets = list(self.plan.sino_offsets)
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def pack_images(self, images):
+        return torch.cat([im.reshape(-1) for im in images]).contiguous()
+
+    def pack_sinos(self, sinos):
+        return torch.cat([s.reshape(-1) for s in sinos]).contiguous()
+
+    def unpack_images(self, flat):
+        return [flat[o:o + h * w].view(h, w) for o, (h, w, _, _) in zip(self.image_offsets, self.shapes)]
+
+    def unpack_sinos(self, flat):
+        return [flat[o:o + v * d].view(v, d) for o, (_, _, v, d) in zip(self.sino_offsets, self.shapes)]
+
+    def projection(self, flat_images):
+        return _PlannedOp.apply(flat_images, self.plan, rt.PROJ, rt.PROJ_T)
+
+    def projection_t(self, flat_sinos):
+        return _PlannedOp.apply(flat_sinos, self.plan, rt.PROJ_T, rt.PROJ)
+
+    def forward(self, flat_images):
+        return self.projection(flat_images)
 
 
 def data_consistency_grad(proj, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:

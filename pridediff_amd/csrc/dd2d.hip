@@ -21,296 +21,61 @@
 #include <cstdio>
 
 #include "dd.h"
+#include "dd2d_body.h"
 #include "dd_common.h"
 #include "dd_profile.h"
 
 namespace dd {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kTile = 16;  // pixel tile edge of k_pixels (16 x 16 = 256)
-
-enum WMode { W_FP = 0, W_BP = 1, W_BPW = 2 };
+using namespace body2d;
 
 int g_forced_tile = 0;
-
-struct Geo2 {
-  int W, H, D, V, B, Bp, BT, Lmax, scan;
-  float dx, dy, det, sid, sdd, shx, shy, bin;
-};
+int g_adjoint_variant = DD_ADJ_PER_PIXEL;
 
 // ---------------------------------------------------------------------------
-// 1. per-view setup (FP64 boundary mapping)
+// one-call kernels: thin grids over the shared bodies
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_setup2d(
     Geo2 g, const float* __restrict__ angles, ViewGeo* __restrict__ views,
     double* __restrict__ edges, float* __restrict__ wfp,
     LineGeo* __restrict__ lines, float* __restrict__ pint) {
-  const int v = blockIdx.x;
-  const float thf = angles[v];
-  const double th = static_cast<double>(thf);
-  int xaxis, flip;
-  choose_axis(thf, &xaxis, &flip);
-  const double s = sin(th), c = cos(th);
-  const double sid = g.sid, sdd = g.sdd, det = g.det, bin = g.bin;
-  const double dx = g.dx, dy = g.dy, shx = g.shx, shy = g.shy;
-  const double sx = -s * sid, sy = c * sid;
-  const int D = g.D;
-  if (threadIdx.x == 0) {
-    ViewGeo vg;
-    vg.sx = sx;
-    vg.sy = sy;
-    vg.s = s;
-    vg.c = c;
-    vg.theta_ref = th + bin;
-    vg.xaxis = xaxis;
-    vg.flip = flip;
-    vg.sxf = static_cast<float>(sx);
-    vg.syf = static_cast<float>(sy);
-    views[v] = vg;
-  }
-  // Bin edges, walk-ordered (FanArcDisCPU2d.cpp:77-86; FanFlatDisCPU2d.cpp:
-  // 72-84; ParaDisCPU2d.cpp:51-55).
-  double* ev = edges + static_cast<size_t>(v) * (D + 1);
-  for (int k = threadIdx.x; k <= D; k += blockDim.x) {
-    const double off = flip ? (D / 2.0 - k) : (k - D / 2.0);
-    double e;
-    if (g.scan == DD_FAN_ARC) {
-      const double t = tan(off * det + bin + th);
-      e = xaxis ? sx + sy * t : sy + sx / t;
-    } else if (g.scan == DD_FAN_FLAT) {
-      const double u = off * (sid / sdd * det) + sid / sdd * bin;
-      const double px = u * c, py = u * s;
-      e = xaxis ? (sx * py - sy * px) / (py - sy)
-                : (sy * px - sx * py) / (px - sx);
-    } else {
-      e = (off * det + bin) / (xaxis ? c : s);
-    }
-    ev[k] = e;
-  }
-  __syncthreads();
-  // FP weight: perpendicular pixel size / (bin width on the axis * cosine of
-  // the bin-centre ray to the axis normal) (FanArcDisCPU2d.cpp:87-93);
-  // parallel beam dy/det or dx/det (ParaDisCPU2d.cpp:60,113).
-  const double perp = xaxis ? dy : dx;
-  for (int k = threadIdx.x; k < D; k += blockDim.x) {
-    double w;
-    if (g.scan == DD_PARA) {
-      w = perp / det;
-    } else {
-      const double e0 = ev[k], e1 = ev[k + 1];
-      const double mid = 0.5 * (e0 + e1);
-      const double cs = xaxis ? fabs(sy) / sqrt((mid - sx) * (mid - sx) + sy * sy)
-                              : fabs(sx) / sqrt((mid - sy) * (mid - sy) + sx * sx);
-      w = perp / ((e1 - e0) * cs);
-    }
-    wfp[static_cast<size_t>(v) * D + k] = static_cast<float>(w);
-  }
-  // Pixel edges of each line on the axis (FanArcDisCPU2d.cpp:95-101,165-171;
-  // ParaDisCPU2d.cpp:57-64,110-117).
-  const int L = xaxis ? g.H : g.W;
-  for (int l = threadIdx.x; l < L; l += blockDim.x) {
-    double a0, p;
-    if (xaxis) {
-      const double yl = (g.H / 2.0 - l - 0.5) * dy + shy;
-      const double x0 = -g.W / 2.0 * dx + shx, x1 = g.W / 2.0 * dx + shx;
-      if (g.scan == DD_PARA) {
-        a0 = x0 + yl * s / c;
-        p = dx;
-      } else {
-        a0 = (sx * yl - sy * x0) / (yl - sy);
-        const double a1 = (sx * yl - sy * x1) / (yl - sy);
-        p = (a1 - a0) / g.W;
-      }
-    } else {
-      const double xl = (l - g.W / 2.0 + 0.5) * dx + shx;
-      const double y0 = -g.H / 2.0 * dy + shy, y1 = g.H / 2.0 * dy + shy;
-      if (g.scan == DD_PARA) {
-        a0 = y0 + xl * c / s;
-        p = dy;
-      } else {
-        a0 = (sy * xl - sx * y0) / (xl - sx);
-        const double a1 = (sy * xl - sx * y1) / (xl - sx);
-        p = (a1 - a0) / g.H;
-      }
-    }
-    LineGeo lg;
-    lg.a0 = a0;
-    lg.inv_p = 1.0 / p;
-    lines[static_cast<size_t>(v) * g.Lmax + l] = lg;
-    pint[static_cast<size_t>(v) * g.Lmax + l] = static_cast<float>(p);
-  }
+  setup2d_view(g, angles, views, edges, wfp, lines, pint, blockIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// 2. layout prologues
-// ---------------------------------------------------------------------------
-// x [B][H][W] -> xrow [H][W][Bp] (lines = rows) and
-//               xcol [W][H][Bp] with xcol[l][i] = x[H-1-i][l] (lines = columns,
-//               pixel index from the bottom, FanArcDisCPU2d.cpp:194).
 template <int BT>
 __global__ __launch_bounds__(kThreads) void k_interleave_image(
     Geo2 g, const float* __restrict__ x, float* __restrict__ xrow,
     float* __restrict__ xcol) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  const int h = blockIdx.y;
-  const int b0 = blockIdx.z * BT;
   if (w >= g.W) return;
-  float v[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b)
-    v[b] = (b0 + b < g.B)
-               ? x[(static_cast<size_t>(b0 + b) * g.H + h) * g.W + w]
-               : 0.0f;
-  store_tile<BT>(xrow + (static_cast<size_t>(h) * g.W + w) * g.Bp + b0, v);
-  store_tile<BT>(xcol + (static_cast<size_t>(w) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
+  interleave_image_px<BT>(g, x, xrow, xcol, blockIdx.y, w, blockIdx.z * BT);
 }
 
-// p [B][V][D] -> s [V][D][Bp] in walk order, optionally scaled by the FP
-// weight of the bin (the adjoint's left factor).
 template <int BT, bool FOLD_W>
 __global__ __launch_bounds__(kThreads) void k_interleave_sino(
     Geo2 g, const ViewGeo* __restrict__ views, const float* __restrict__ wfp,
     const float* __restrict__ p, float* __restrict__ s) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int v = blockIdx.y;
-  const int b0 = blockIdx.z * BT;
   if (k >= g.D) return;
-  const int kk = views[v].flip ? g.D - 1 - k : k;
-  const float wk = FOLD_W ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
-  float val[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b)
-    val[b] = (b0 + b < g.B)
-                 ? p[(static_cast<size_t>(b0 + b) * g.V + v) * g.D + kk] * wk
-                 : 0.0f;
-  store_tile<BT>(s + (static_cast<size_t>(v) * g.D + k) * g.Bp + b0, val);
+  interleave_sino_bin<BT, FOLD_W>(g, views, wfp, p, s, blockIdx.y, k, blockIdx.z * BT);
 }
 
-// ---------------------------------------------------------------------------
-// weights shared by both gather kernels
-// ---------------------------------------------------------------------------
-// BackProj constant: arc SID*dgamma (FanArcDisCPU2d.cpp:451), flat virtual
-// pitch (FanFlatDisCPU2d.cpp:478), parallel det (ParaDisCPU2d.cpp:202); the
-// reference divides it by the pixel interval and multiplies by the overlap on
-// the axis, which is this constant times the overlap in pixel units.
-__host__ __device__ inline float bp_const(const Geo2& g) {
-  if (g.scan == DD_FAN_ARC) return g.sid * g.det;
-  if (g.scan == DD_FAN_FLAT) return g.sid / g.sdd * g.det;
-  return g.det;
-}
-
-// FBP distance weight of a pixel centre: arc 1/|S-P|^2
-// (FanArcDisCPU2d.cpp:27-29,507), flat (r/(r-d))^2 (FanFlatDisCPU2d.cpp:19-22).
-__device__ __forceinline__ float fbp_weight(const Geo2& g, const ViewGeo& vg,
-                                            float xc, float yc) {
-  if (g.scan == DD_FAN_ARC) {
-    const float ddx = vg.sxf - xc, ddy = vg.syf - yc;
-    return 1.0f / (ddx * ddx + ddy * ddy);
-  }
-  const float r = g.sid;
-  const float d = (vg.sxf * xc + vg.syf * yc) / r;
-  return (r * r) / ((r - d) * (r - d));
-}
-
-// Pixel centre of (line, pixel) in world coordinates.
-__device__ __forceinline__ void pixel_centre(const Geo2& g, int xaxis, int l,
-                                             int i, float* xc, float* yc) {
-  if (xaxis) {
-    *xc = (i - g.W * 0.5f + 0.5f) * g.dx + g.shx;
-    *yc = (g.H * 0.5f - l - 0.5f) * g.dy + g.shy;
-  } else {
-    *xc = (l - g.W * 0.5f + 0.5f) * g.dx + g.shx;
-    *yc = (i - g.H * 0.5f + 0.5f) * g.dy + g.shy;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// 3a. image -> sinogram: one thread per (view, bin), BT instances
-// ---------------------------------------------------------------------------
+// image -> sinogram: one thread per (view, bin), BT instances
 template <int BT, int WM>
 __global__ __launch_bounds__(kThreads) void k_bins(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const float* __restrict__ wfp, const LineGeo* __restrict__ lines,
     const float* __restrict__ pint, const float* __restrict__ xrow,
     const float* __restrict__ xcol, float* __restrict__ out) {
-  const int v = blockIdx.x;
   const int k = blockIdx.y * blockDim.x + threadIdx.x;
-  const int b0 = blockIdx.z * BT;
   if (k >= g.D) return;
-  const ViewGeo vg = views[v];
-  const int xaxis = vg.xaxis;
-  const int L = xaxis ? g.H : g.W;
-  const int N = xaxis ? g.W : g.H;
-  const float* __restrict__ src = (xaxis ? xrow : xcol) + b0;
-  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + k];
-  const double e2 = edges[static_cast<size_t>(v) * (g.D + 1) + k + 1];
-  const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
-  const float* __restrict__ pv = pint + static_cast<size_t>(v) * g.Lmax;
-  const float kbp = bp_const(g);
-  float acc[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
-
-  for (int l = 0; l < L; ++l) {
-    const LineGeo lg = lv[l];
-    const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
-    const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
-    const int cs = max(p1.c, 0);
-    const int ce = min(p2.c, N - 1);
-    if (cs > ce) continue;
-    const float lw = (WM == W_FP) ? pv[l] : kbp;
-    const float* __restrict__ line = src + static_cast<size_t>(l) * N * g.Bp;
-    for (int c = cs; c <= ce; ++c) {
-      float w = overlap_in_pixel(p1, p2, c) * lw;
-      if constexpr (WM == W_BPW) {
-        float xc, yc;
-        pixel_centre(g, xaxis, l, c, &xc, &yc);
-        w *= fbp_weight(g, vg, xc, yc);
-      }
-      float xv[BT];
-      load_tile<BT>(line + static_cast<size_t>(c) * g.Bp, xv);
-#pragma unroll
-      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
-    }
-  }
-  const int kk = vg.flip ? g.D - 1 - k : k;
-  const float wk = (WM == W_FP) ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
-#pragma unroll
-  for (int b = 0; b < BT; ++b)
-    if (b0 + b < g.B)
-      out[(static_cast<size_t>(b0 + b) * g.V + v) * g.D + kk] = acc[b] * wk;
+  bins_body<BT, WM>(g, views, edges, wfp, lines, pint, xrow, xcol, out,
+                    blockIdx.x, k, blockIdx.z * BT);
 }
 
-// ---------------------------------------------------------------------------
-// 3b. sinogram -> image: one thread per pixel, BT instances, all views
-// ---------------------------------------------------------------------------
-// First walk-ordered bin worth testing for an axis point: the inverse of the
-// edge map, closed form per geometry.  Only a starting guess; the exact range
-// is then fixed against the edge table with the shared edge_pos().
-__device__ __forceinline__ int guess_bin(const Geo2& g, const ViewGeo& vg,
-                                         double a) {
-  double kf;
-  if (g.scan == DD_FAN_ARC) {
-    double gam = vg.xaxis ? atan2(a - vg.sx, vg.sy) : atan2(vg.sx, a - vg.sy);
-    double d = gam - vg.theta_ref;
-    d -= M_PI * rint(d / M_PI);
-    kf = g.D * 0.5 + (vg.flip ? -d : d) / static_cast<double>(g.det);
-  } else if (g.scan == DD_FAN_FLAT) {
-    const double R = g.sid;
-    const double u = vg.xaxis ? R * a * vg.c / (R + vg.s * a)
-                              : R * a * vg.s / (R - vg.c * a);
-    const double vdet = static_cast<double>(g.sid) / g.sdd * g.det;
-    const double vbin = static_cast<double>(g.sid) / g.sdd * g.bin;
-    kf = g.D * 0.5 + (vg.flip ? -(u - vbin) : (u - vbin)) / vdet;
-  } else {
-    const double u = vg.xaxis ? a * vg.c : a * vg.s;
-    kf = g.D * 0.5 + (vg.flip ? -(u - g.bin) : (u - g.bin)) / static_cast<double>(g.det);
-  }
-  kf = fmin(fmax(kf, 0.0), g.D - 1.0);
-  return static_cast<int>(kf);
-}
-
+// sinogram -> image: one thread per pixel, BT instances, all views.
+// Variant DD_ADJ_PER_PIXEL: every pixel finds its bins on its own.
 template <int BT, int WM>
 __global__ __launch_bounds__(kThreads) void k_pixels(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
@@ -318,53 +83,20 @@ __global__ __launch_bounds__(kThreads) void k_pixels(
     const float* __restrict__ sino, float* __restrict__ out) {
   const int c = blockIdx.x * kTile + (threadIdx.x % kTile);
   const int r = blockIdx.y * kTile + (threadIdx.x / kTile);
-  const int b0 = blockIdx.z * BT;
   if (c >= g.W || r >= g.H) return;
-  const float kbp = bp_const(g);
-  float xcw = 0.0f, ycw = 0.0f;
-  if constexpr (WM == W_BPW) pixel_centre(g, 1, r, c, &xcw, &ycw);
-  float acc[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  pixels_body<BT, WM>(g, views, edges, lines, pint, sino, out, r, c, blockIdx.z * BT);
+}
 
-  for (int v = 0; v < g.V; ++v) {
-    const ViewGeo vg = views[v];
-    int l, i, N;
-    if (vg.xaxis) {
-      l = r;
-      i = c;
-      N = g.W;
-    } else {
-      l = c;
-      i = g.H - 1 - r;
-      N = g.H;
-    }
-    const LineGeo lg = lines[static_cast<size_t>(v) * g.Lmax + l];
-    const float pl = pint[static_cast<size_t>(v) * g.Lmax + l];
-    const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
-    int k = guess_bin(g, vg, lg.a0 + (i + 0.5) * static_cast<double>(pl));
-    while (k > 0 && edge_pos(ev[k], lg.a0, lg.inv_p, N).c >= i) --k;
-    while (k < g.D && edge_pos(ev[k + 1], lg.a0, lg.inv_p, N).c < i) ++k;
-    if (k >= g.D) continue;
-    float gw = (WM == W_FP) ? pl : kbp;
-    if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
-    EdgePos p1 = edge_pos(ev[k], lg.a0, lg.inv_p, N);
-    const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
-    while (p1.c <= i) {
-      const EdgePos p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
-      const float w = overlap_in_pixel(p1, p2, i) * gw;
-      float q[BT];
-      load_tile<BT>(sv + static_cast<size_t>(k) * g.Bp, q);
-#pragma unroll
-      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
-      p1 = p2;
-      if (++k >= g.D) break;
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < BT; ++b)
-    if (b0 + b < g.B)
-      out[(static_cast<size_t>(b0 + b) * g.H + r) * g.W + c] = acc[b];
+// Variants DD_ADJ_TILE16 / DD_ADJ_TILE8: a T x T tile stages its lines' edge
+// positions per view in LDS.
+template <int BT, int WM, int T>
+__global__ __launch_bounds__(T * T) void k_pixels_tile(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out) {
+  __shared__ PixelTileLds<T> lds;
+  pixels_tile_body<BT, WM, T>(g, views, edges, lines, pint, sino, out, blockIdx.x,
+                              blockIdx.y, blockIdx.z * BT, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,8 +251,8 @@ void launch_setup(const Geo2& g, const float* angles, void* geo,
 }
 
 template <int BT>
-dd_status launch_tile(const Geo2& g, int op, const float* input, float* output,
-                      const GeoPtrs& G, char* ws, const Layout& L,
+dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
+                      float* output, const GeoPtrs& G, char* ws, const Layout& L,
                       hipStream_t st) {
   auto* buf0 = reinterpret_cast<float*>(ws + L.buf0);
   auto* buf1 = reinterpret_cast<float*>(ws + L.buf1);
@@ -548,29 +280,54 @@ dd_status launch_tile(const Geo2& g, int op, const float* input, float* output,
     else
       hipLaunchKernelGGL((k_interleave_sino<BT, false>), sgrid, dim3(kThreads),
                          0, st, g, G.views, G.wfp, input, buf0);
-    dim3 grid((g.W + kTile - 1) / kTile, (g.H + kTile - 1) / kTile, ntile);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (op == DD_PROJ_T)
-      hipLaunchKernelGGL((k_pixels<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                         G.views, G.edges, G.lines, G.pint, buf0, output);
-    else if (op == DD_BPROJ)
-      hipLaunchKernelGGL((k_pixels<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                         G.views, G.edges, G.lines, G.pint, buf0, output);
-    else
-      hipLaunchKernelGGL((k_pixels<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                         G.views, G.edges, G.lines, G.pint, buf0, output);
+    if (variant == DD_ADJ_TILE16) {
+      dim3 grid((g.W + 15) / 16, (g.H + 15) / 16, ntile);
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 16>), grid, dim3(256), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else if (op == DD_BPROJ)
+        hipLaunchKernelGGL((k_pixels_tile<BT, W_BP, 16>), grid, dim3(256), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else
+        hipLaunchKernelGGL((k_pixels_tile<BT, W_BPW, 16>), grid, dim3(256), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+    } else if (variant == DD_ADJ_TILE8) {
+      dim3 grid((g.W + 7) / 8, (g.H + 7) / 8, ntile);
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 8>), grid, dim3(64), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else if (op == DD_BPROJ)
+        hipLaunchKernelGGL((k_pixels_tile<BT, W_BP, 8>), grid, dim3(64), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else
+        hipLaunchKernelGGL((k_pixels_tile<BT, W_BPW, 8>), grid, dim3(64), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+    } else {
+      dim3 grid((g.W + kTile - 1) / kTile, (g.H + kTile - 1) / kTile, ntile);
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL((k_pixels<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else if (op == DD_BPROJ)
+        hipLaunchKernelGGL((k_pixels<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else
+        hipLaunchKernelGGL((k_pixels<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+    }
   }
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
 }
 
-dd_status run_tiles(const Geo2& q, int op, const float* input, float* output,
-                    const GeoPtrs& G, char* w, const Layout& L, hipStream_t st) {
+dd_status run_tiles(const Geo2& q, int op, int variant, const float* input,
+                    float* output, const GeoPtrs& G, char* w, const Layout& L,
+                    hipStream_t st) {
   switch (q.BT) {
-    case 1: return launch_tile<1>(q, op, input, output, G, w, L, st);
-    case 2: return launch_tile<2>(q, op, input, output, G, w, L, st);
-    case 4: return launch_tile<4>(q, op, input, output, G, w, L, st);
-    case 8: return launch_tile<8>(q, op, input, output, G, w, L, st);
-    default: return launch_tile<16>(q, op, input, output, G, w, L, st);
+    case 1: return launch_tile<1>(q, op, variant, input, output, G, w, L, st);
+    case 2: return launch_tile<2>(q, op, variant, input, output, G, w, L, st);
+    case 4: return launch_tile<4>(q, op, variant, input, output, G, w, L, st);
+    case 8: return launch_tile<8>(q, op, variant, input, output, G, w, L, st);
+    default: return launch_tile<16>(q, op, variant, input, output, G, w, L, st);
   }
 }
 
@@ -589,6 +346,11 @@ using namespace dd;
 extern "C" {
 
 const char* dd_version(void) { return "libdd 0.1 (gfx950)"; }
+
+void dd2d_set_adjoint_variant(int variant) {
+  g_adjoint_variant = (variant >= DD_ADJ_PER_PIXEL && variant <= DD_ADJ_TILE8) ? variant
+                                                                               : DD_ADJ_PER_PIXEL;
+}
 
 void dd2d_set_instance_tile(int tile) {
   g_forced_tile = (tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16) ? tile : 0;
@@ -615,7 +377,8 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
   launch_setup(q, angles, w, L, st);
-  return run_tiles(q, op, input, output, geo_ptrs(w, L), w + L.geo_total, L, st);
+  return run_tiles(q, op, g_adjoint_variant, input, output, geo_ptrs(w, L), w + L.geo_total,
+                   L, st);
 }
 
 size_t dd2d_geometry_size(int scan, const dd_geom2d* g, int views) {
@@ -636,25 +399,31 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
 }
 
+static int valid_tile(int tile, int batch) {
+  return (tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16) ? tile
+                                                                          : pick_tile(batch);
+}
+
 size_t dd2d_planned_workspace_size(int scan, int op, const dd_geom2d* g,
-                                   int batch, int views) {
+                                   int batch, int views, int tile) {
   Geo2 q;
-  if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return 0;
+  if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return 0;
   return make_layout(q, op).data_total;
 }
 
 dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
                            const void* geo, const float* input, float* output,
-                           int batch, int views, void* ws, size_t ws_bytes,
-                           void* stream) {
+                           int batch, int views, int tile, int variant, void* ws,
+                           size_t ws_bytes, void* stream) {
   dd_status rc = check_op(scan, op);
   if (rc != DD_OK) return rc;
   if (!geo || !input || !output) return DD_BAD_ARG;
   Geo2 q;
-  if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
+  if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return DD_BAD_ARG;
   const Layout L = make_layout(q, op);
   if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
-  return run_tiles(q, op, input, output, geo_ptrs(geo, L),
+  if (variant < DD_ADJ_PER_PIXEL || variant > DD_ADJ_TILE8) variant = g_adjoint_variant;
+  return run_tiles(q, op, variant, input, output, geo_ptrs(geo, L),
                    static_cast<char*>(ws), L, static_cast<hipStream_t>(stream));
 }
 

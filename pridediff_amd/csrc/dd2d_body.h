@@ -1,0 +1,512 @@
+// Device-side bodies of the 2D distance-driven kernels, shared by the
+// one-call kernels (dd2d.hip) and the persistent batched kernels
+// (ddbatch.hip), so both paths execute the same arithmetic.
+//
+// Behaviour followed: deeprecon/cpu/dis2d/FanArcDisCPU2d.cpp (fan arc),
+// FanFlatDisCPU2d.cpp (fan flat), ParaDisCPU2d.cpp (parallel).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dd.h"
+#include "dd_common.h"
+
+namespace dd {
+namespace body2d {
+
+
+constexpr int kThreads = 256;
+constexpr int kTile = 16;  // pixel tile edge of k_pixels (16 x 16 = 256)
+
+enum WMode { W_FP = 0, W_BP = 1, W_BPW = 2 };
+
+struct Geo2 {
+  int W, H, D, V, B, Bp, BT, Lmax, scan;
+  float dx, dy, det, sid, sdd, shx, shy, bin;
+};
+
+// ---------------------------------------------------------------------------
+// 1. per-view setup (FP64 boundary mapping)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void setup2d_view(
+    const Geo2& g, const float* __restrict__ angles, ViewGeo* __restrict__ views,
+    double* __restrict__ edges, float* __restrict__ wfp,
+    LineGeo* __restrict__ lines, float* __restrict__ pint, int v) {
+  const float thf = angles[v];
+  const double th = static_cast<double>(thf);
+  int xaxis, flip;
+  choose_axis(thf, &xaxis, &flip);
+  const double s = sin(th), c = cos(th);
+  const double sid = g.sid, sdd = g.sdd, det = g.det, bin = g.bin;
+  const double dx = g.dx, dy = g.dy, shx = g.shx, shy = g.shy;
+  const double sx = -s * sid, sy = c * sid;
+  const int D = g.D;
+  if (threadIdx.x == 0) {
+    ViewGeo vg;
+    vg.sx = sx;
+    vg.sy = sy;
+    vg.s = s;
+    vg.c = c;
+    vg.theta_ref = th + bin;
+    vg.xaxis = xaxis;
+    vg.flip = flip;
+    vg.sxf = static_cast<float>(sx);
+    vg.syf = static_cast<float>(sy);
+    views[v] = vg;
+  }
+  // Bin edges, walk-ordered (FanArcDisCPU2d.cpp:77-86; FanFlatDisCPU2d.cpp:
+  // 72-84; ParaDisCPU2d.cpp:51-55).
+  double* ev = edges + static_cast<size_t>(v) * (D + 1);
+  for (int k = threadIdx.x; k <= D; k += blockDim.x) {
+    const double off = flip ? (D / 2.0 - k) : (k - D / 2.0);
+    double e;
+    if (g.scan == DD_FAN_ARC) {
+      const double t = tan(off * det + bin + th);
+      e = xaxis ? sx + sy * t : sy + sx / t;
+    } else if (g.scan == DD_FAN_FLAT) {
+      const double u = off * (sid / sdd * det) + sid / sdd * bin;
+      const double px = u * c, py = u * s;
+      e = xaxis ? (sx * py - sy * px) / (py - sy)
+                : (sy * px - sx * py) / (px - sx);
+    } else {
+      e = (off * det + bin) / (xaxis ? c : s);
+    }
+    ev[k] = e;
+  }
+  __syncthreads();
+  // FP weight: perpendicular pixel size / (bin width on the axis * cosine of
+  // the bin-centre ray to the axis normal) (FanArcDisCPU2d.cpp:87-93);
+  // parallel beam dy/det or dx/det (ParaDisCPU2d.cpp:60,113).
+  const double perp = xaxis ? dy : dx;
+  for (int k = threadIdx.x; k < D; k += blockDim.x) {
+    double w;
+    if (g.scan == DD_PARA) {
+      w = perp / det;
+    } else {
+      const double e0 = ev[k], e1 = ev[k + 1];
+      const double mid = 0.5 * (e0 + e1);
+      const double cs = xaxis ? fabs(sy) / sqrt((mid - sx) * (mid - sx) + sy * sy)
+                              : fabs(sx) / sqrt((mid - sy) * (mid - sy) + sx * sx);
+      w = perp / ((e1 - e0) * cs);
+    }
+    wfp[static_cast<size_t>(v) * D + k] = static_cast<float>(w);
+  }
+  // Pixel edges of each line on the axis (FanArcDisCPU2d.cpp:95-101,165-171;
+  // ParaDisCPU2d.cpp:57-64,110-117).
+  const int L = xaxis ? g.H : g.W;
+  for (int l = threadIdx.x; l < L; l += blockDim.x) {
+    double a0, p;
+    if (xaxis) {
+      const double yl = (g.H / 2.0 - l - 0.5) * dy + shy;
+      const double x0 = -g.W / 2.0 * dx + shx, x1 = g.W / 2.0 * dx + shx;
+      if (g.scan == DD_PARA) {
+        a0 = x0 + yl * s / c;
+        p = dx;
+      } else {
+        a0 = (sx * yl - sy * x0) / (yl - sy);
+        const double a1 = (sx * yl - sy * x1) / (yl - sy);
+        p = (a1 - a0) / g.W;
+      }
+    } else {
+      const double xl = (l - g.W / 2.0 + 0.5) * dx + shx;
+      const double y0 = -g.H / 2.0 * dy + shy, y1 = g.H / 2.0 * dy + shy;
+      if (g.scan == DD_PARA) {
+        a0 = y0 + xl * c / s;
+        p = dy;
+      } else {
+        a0 = (sy * xl - sx * y0) / (xl - sx);
+        const double a1 = (sy * xl - sx * y1) / (xl - sx);
+        p = (a1 - a0) / g.H;
+      }
+    }
+    LineGeo lg;
+    lg.a0 = a0;
+    lg.inv_p = 1.0 / p;
+    lines[static_cast<size_t>(v) * g.Lmax + l] = lg;
+    pint[static_cast<size_t>(v) * g.Lmax + l] = static_cast<float>(p);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. layout prologues
+// ---------------------------------------------------------------------------
+// x [B][H][W] -> xrow [H][W][Bp] (lines = rows) and
+//               xcol [W][H][Bp] with xcol[l][i] = x[H-1-i][l] (lines = columns,
+//               pixel index from the bottom, FanArcDisCPU2d.cpp:194).
+template <int BT>
+__device__ __forceinline__ void interleave_image_px(
+    const Geo2& g, const float* __restrict__ x, float* __restrict__ xrow,
+    float* __restrict__ xcol, int h, int w, int b0) {
+  float v[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    v[b] = (b0 + b < g.B)
+               ? x[(static_cast<size_t>(b0 + b) * g.H + h) * g.W + w]
+               : 0.0f;
+  store_tile<BT>(xrow + (static_cast<size_t>(h) * g.W + w) * g.Bp + b0, v);
+  store_tile<BT>(xcol + (static_cast<size_t>(w) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
+}
+
+// p [B][V][D] -> s [V][D][Bp] in walk order, optionally scaled by the FP
+// weight of the bin (the adjoint's left factor).
+template <int BT, bool FOLD_W>
+__device__ __forceinline__ void interleave_sino_bin(
+    const Geo2& g, const ViewGeo* __restrict__ views, const float* __restrict__ wfp,
+    const float* __restrict__ p, float* __restrict__ s, int v, int k, int b0) {
+  const int kk = views[v].flip ? g.D - 1 - k : k;
+  const float wk = FOLD_W ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
+  float val[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    val[b] = (b0 + b < g.B)
+                 ? p[(static_cast<size_t>(b0 + b) * g.V + v) * g.D + kk] * wk
+                 : 0.0f;
+  store_tile<BT>(s + (static_cast<size_t>(v) * g.D + k) * g.Bp + b0, val);
+}
+
+// ---------------------------------------------------------------------------
+// weights shared by both gather kernels
+// ---------------------------------------------------------------------------
+// BackProj constant: arc SID*dgamma (FanArcDisCPU2d.cpp:451), flat virtual
+// pitch (FanFlatDisCPU2d.cpp:478), parallel det (ParaDisCPU2d.cpp:202); the
+// reference divides it by the pixel interval and multiplies by the overlap on
+// the axis, which is this constant times the overlap in pixel units.
+__host__ __device__ __forceinline__ float bp_const(const Geo2& g) {
+  if (g.scan == DD_FAN_ARC) return g.sid * g.det;
+  if (g.scan == DD_FAN_FLAT) return g.sid / g.sdd * g.det;
+  return g.det;
+}
+
+// FBP distance weight of a pixel centre: arc 1/|S-P|^2
+// (FanArcDisCPU2d.cpp:27-29,507), flat (r/(r-d))^2 (FanFlatDisCPU2d.cpp:19-22).
+__device__ __forceinline__ float fbp_weight(const Geo2& g, const ViewGeo& vg,
+                                            float xc, float yc) {
+  if (g.scan == DD_FAN_ARC) {
+    const float ddx = vg.sxf - xc, ddy = vg.syf - yc;
+    return 1.0f / (ddx * ddx + ddy * ddy);
+  }
+  const float r = g.sid;
+  const float d = (vg.sxf * xc + vg.syf * yc) / r;
+  return (r * r) / ((r - d) * (r - d));
+}
+
+// Pixel centre of (line, pixel) in world coordinates.
+__device__ __forceinline__ void pixel_centre(const Geo2& g, int xaxis, int l,
+                                             int i, float* xc, float* yc) {
+  if (xaxis) {
+    *xc = (i - g.W * 0.5f + 0.5f) * g.dx + g.shx;
+    *yc = (g.H * 0.5f - l - 0.5f) * g.dy + g.shy;
+  } else {
+    *xc = (l - g.W * 0.5f + 0.5f) * g.dx + g.shx;
+    *yc = (i - g.H * 0.5f + 0.5f) * g.dy + g.shy;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3a. image -> sinogram: one thread per (view, bin), BT instances
+// ---------------------------------------------------------------------------
+template <int BT, int WM>
+__device__ __forceinline__ void bins_body(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const float* __restrict__ wfp, const LineGeo* __restrict__ lines,
+    const float* __restrict__ pint, const float* __restrict__ xrow,
+    const float* __restrict__ xcol, float* __restrict__ out, int v, int k, int b0) {
+  const ViewGeo vg = views[v];
+  const int xaxis = vg.xaxis;
+  const int L = xaxis ? g.H : g.W;
+  const int N = xaxis ? g.W : g.H;
+  const float* __restrict__ src = (xaxis ? xrow : xcol) + b0;
+  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + k];
+  const double e2 = edges[static_cast<size_t>(v) * (g.D + 1) + k + 1];
+  const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
+  const float* __restrict__ pv = pint + static_cast<size_t>(v) * g.Lmax;
+  const float kbp = bp_const(g);
+  float acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+
+  for (int l = 0; l < L; ++l) {
+    const LineGeo lg = lv[l];
+    const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
+    const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
+    const int cs = max(p1.c, 0);
+    const int ce = min(p2.c, N - 1);
+    if (cs > ce) continue;
+    const float lw = (WM == W_FP) ? pv[l] : kbp;
+    const float* __restrict__ line = src + static_cast<size_t>(l) * N * g.Bp;
+    for (int c = cs; c <= ce; ++c) {
+      float w = overlap_in_pixel(p1, p2, c) * lw;
+      if constexpr (WM == W_BPW) {
+        float xc, yc;
+        pixel_centre(g, xaxis, l, c, &xc, &yc);
+        w *= fbp_weight(g, vg, xc, yc);
+      }
+      float xv[BT];
+      load_tile<BT>(line + static_cast<size_t>(c) * g.Bp, xv);
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
+    }
+  }
+  const int kk = vg.flip ? g.D - 1 - k : k;
+  const float wk = (WM == W_FP) ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    if (b0 + b < g.B)
+      out[(static_cast<size_t>(b0 + b) * g.V + v) * g.D + kk] = acc[b] * wk;
+}
+
+// ---------------------------------------------------------------------------
+// 3b. sinogram -> image: one thread per pixel, BT instances, all views
+// ---------------------------------------------------------------------------
+// First walk-ordered bin worth testing for an axis point: the inverse of the
+// edge map, closed form per geometry.  Only a starting guess; the exact range
+// is then fixed against the edge table with the shared edge_pos().
+__device__ __forceinline__ int guess_bin(const Geo2& g, const ViewGeo& vg,
+                                         double a) {
+  double kf;
+  if (g.scan == DD_FAN_ARC) {
+    double gam = vg.xaxis ? atan2(a - vg.sx, vg.sy) : atan2(vg.sx, a - vg.sy);
+    double d = gam - vg.theta_ref;
+    d -= M_PI * rint(d / M_PI);
+    kf = g.D * 0.5 + (vg.flip ? -d : d) / static_cast<double>(g.det);
+  } else if (g.scan == DD_FAN_FLAT) {
+    const double R = g.sid;
+    const double u = vg.xaxis ? R * a * vg.c / (R + vg.s * a)
+                              : R * a * vg.s / (R - vg.c * a);
+    const double vdet = static_cast<double>(g.sid) / g.sdd * g.det;
+    const double vbin = static_cast<double>(g.sid) / g.sdd * g.bin;
+    kf = g.D * 0.5 + (vg.flip ? -(u - vbin) : (u - vbin)) / vdet;
+  } else {
+    const double u = vg.xaxis ? a * vg.c : a * vg.s;
+    kf = g.D * 0.5 + (vg.flip ? -(u - g.bin) : (u - g.bin)) / static_cast<double>(g.det);
+  }
+  kf = fmin(fmax(kf, 0.0), g.D - 1.0);
+  return static_cast<int>(kf);
+}
+
+// Cheap first-bin guess in FP32 (a starting window only; exactness comes
+// from edge_pos and the window checks in pixels_tile_body).
+__device__ __forceinline__ int guess_bin_fast(const Geo2& g, const ViewGeo& vg,
+                                              double a) {
+  float kf;
+  if (g.scan == DD_FAN_ARC) {
+    const float gam = vg.xaxis ? atan2f(static_cast<float>(a - vg.sx), static_cast<float>(vg.sy))
+                               : atan2f(static_cast<float>(vg.sx), static_cast<float>(a - vg.sy));
+    float d = gam - static_cast<float>(vg.theta_ref);
+    d -= 3.14159265f * rintf(d * 0.318309886f);
+    kf = g.D * 0.5f + (vg.flip ? -d : d) / g.det;
+  } else if (g.scan == DD_FAN_FLAT) {
+    const float R = g.sid, af = static_cast<float>(a);
+    const float sf = static_cast<float>(vg.s), cf = static_cast<float>(vg.c);
+    const float u = vg.xaxis ? R * af * cf / (R + sf * af) : R * af * sf / (R - cf * af);
+    const float vdet = g.sid / g.sdd * g.det, vbin = g.sid / g.sdd * g.bin;
+    kf = g.D * 0.5f + (vg.flip ? -(u - vbin) : (u - vbin)) / vdet;
+  } else {
+    const float u = static_cast<float>(vg.xaxis ? a * vg.c : a * vg.s);
+    kf = g.D * 0.5f + (vg.flip ? -(u - g.bin) : (u - g.bin)) / g.det;
+  }
+  kf = fminf(fmaxf(kf, -4.0f), g.D + 4.0f);
+  return static_cast<int>(floorf(kf));
+}
+
+// Contribution of view v to pixel (r, c): find the bins whose edges bracket
+// the pixel in its line (closed-form guess, fixed exactly with edge_pos),
+// then accumulate overlap * weight * sinogram.
+template <int BT, int WM>
+__device__ __forceinline__ void pixel_one_view(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, int r, int c, int b0, float xcw, float ycw,
+    float (&acc)[BT]) {
+  int l, i, N;
+  if (vg.xaxis) {
+    l = r;
+    i = c;
+    N = g.W;
+  } else {
+    l = c;
+    i = g.H - 1 - r;
+    N = g.H;
+  }
+  const LineGeo lg = lines[static_cast<size_t>(v) * g.Lmax + l];
+  const float pl = pint[static_cast<size_t>(v) * g.Lmax + l];
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  int k = min(max(guess_bin_fast(g, vg, lg.a0 + (i + 0.5) * static_cast<double>(pl)), 0),
+              g.D - 1);
+  while (k > 0 && edge_pos(ev[k], lg.a0, lg.inv_p, N).c >= i) --k;
+  while (k < g.D && edge_pos(ev[k + 1], lg.a0, lg.inv_p, N).c < i) ++k;
+  if (k >= g.D) return;
+  float gw = (WM == W_FP) ? pl : bp_const(g);
+  if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
+  EdgePos p1 = edge_pos(ev[k], lg.a0, lg.inv_p, N);
+  const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+  while (p1.c <= i) {
+    const EdgePos p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+    const float w = overlap_in_pixel(p1, p2, i) * gw;
+    float q[BT];
+    load_tile<BT>(sv + static_cast<size_t>(k) * g.Bp, q);
+#pragma unroll
+    for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
+    p1 = p2;
+    if (++k >= g.D) break;
+  }
+}
+
+// sinogram -> image for one pixel, all views, each view found independently.
+template <int BT, int WM>
+__device__ __forceinline__ void pixels_body(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out, int r, int c, int b0) {
+  float xcw = 0.0f, ycw = 0.0f;
+  if constexpr (WM == W_BPW) pixel_centre(g, 1, r, c, &xcw, &ycw);
+  float acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  for (int v = 0; v < g.V; ++v)
+    pixel_one_view<BT, WM>(g, views[v], v, edges, lines, pint, sino, r, c, b0, xcw, ycw, acc);
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    if (b0 + b < g.B)
+      out[(static_cast<size_t>(b0 + b) * g.H + r) * g.W + c] = acc[b];
+}
+
+// Edge positions staged per (line of a pixel tile, view).
+template <int T>
+struct PixelTileLds {
+  static constexpr int kCap = 4 * T;  // staged edges per line
+  int klo[T];                 // first staged edge of each tile line
+  int n[T];                   // staged edges of each tile line
+  int first[T][T + 1];        // first staged edge with c >= i0 + j
+  int ec[T][kCap];            // edge_pos().c of staged edges
+  float ef[T][kCap];          // edge_pos().f of staged edges
+  int fallback[2];            // per view parity: window did not cover
+};
+
+// sinogram -> image for one kTile x kTile pixel tile, all views; every thread
+// of the block must call it (barriers).  Per view, the tile's lines get
+// their bin-edge positions computed once into LDS (instead of once per
+// pixel), a first-edge table is scattered from them, and each pixel reads
+// its bin range and overlaps from LDS.  The overlaps are the same edge_pos
+// values the forward kernel uses, so A^T stays the exact transpose of A.
+// A line whose staged window misses part of the tile takes the per-pixel
+// path for that view (same arithmetic, slower).
+template <int BT, int WM, int kTile>
+__device__ __forceinline__ void pixels_tile_body(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out, int tile_c, int tile_r,
+    int b0, PixelTileLds<kTile>& s) {
+  constexpr int kEdgeCap = PixelTileLds<kTile>::kCap;
+  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+  const int c0 = tile_c * kTile, r0 = tile_r * kTile;
+  const int c = c0 + tx, r = r0 + ty;
+  const bool inside = (c < g.W) && (r < g.H);
+  float xcw = 0.0f, ycw = 0.0f;
+  if constexpr (WM == W_BPW) pixel_centre(g, 1, r, c, &xcw, &ycw);
+  const float kbp = bp_const(g);
+  float acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  if (threadIdx.x < 2) s.fallback[threadIdx.x] = 0;
+  __syncthreads();
+
+  for (int v = 0; v < g.V; ++v) {
+    const ViewGeo vg = views[v];
+    const int xaxis = vg.xaxis;
+    const int L = xaxis ? g.H : g.W;
+    const int N = xaxis ? g.W : g.H;
+    const int lbase = xaxis ? r0 : c0;
+    const int i0 = xaxis ? c0 : g.H - kTile - r0;
+    const int par = v & 1;
+    const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+    const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
+    // A: staged edge window of each tile line
+    if (threadIdx.x == 0) s.fallback[par ^ 1] = 0;
+    if (threadIdx.x < kTile) {
+      const int la = lbase + threadIdx.x;
+      int klo = 0, n = 0;
+      if (la < L) {
+        const LineGeo lg = lv[la];
+        const double p = static_cast<double>(pint[static_cast<size_t>(v) * g.Lmax + la]);
+        const int k0 = guess_bin_fast(g, vg, lg.a0 + i0 * p) - 2;
+        const int k1 = guess_bin_fast(g, vg, lg.a0 + (i0 + kTile) * p) + 3;
+        klo = min(max(k0, 0), g.D);
+        const int khi = min(max(k1, klo), g.D);
+        n = khi - klo + 1;
+        if (n > kEdgeCap) {
+          n = kEdgeCap;
+          s.fallback[par] = 1;
+        }
+      }
+      s.klo[threadIdx.x] = klo;
+      s.n[threadIdx.x] = n;
+#pragma unroll
+      for (int j = 0; j <= kTile; ++j) s.first[threadIdx.x][j] = n;
+    }
+    __syncthreads();
+    // B: edge positions of every staged edge, and the window checks
+    for (int idx = threadIdx.x; idx < kTile * kEdgeCap; idx += blockDim.x) {
+      const int ll = idx / kEdgeCap, e = idx % kEdgeCap;
+      const int n = s.n[ll];
+      if (e >= n) continue;
+      const int klo = s.klo[ll];
+      const LineGeo lg = lv[lbase + ll];
+      const EdgePos ep = edge_pos(ev[klo + e], lg.a0, lg.inv_p, N);
+      s.ec[ll][e] = ep.c;
+      s.ef[ll][e] = ep.f;
+      // the window must start before the tile's first real pixel and end
+      // after its last one, unless it reaches the detector's end
+      if (e == 0 && klo > 0 && ep.c >= max(i0, 0)) s.fallback[par] = 1;
+      if (e == n - 1 && klo + e < g.D && ep.c < min(i0 + kTile, N)) s.fallback[par] = 1;
+    }
+    __syncthreads();
+    // C: first staged edge at or after each pixel edge of the tile
+    for (int idx = threadIdx.x; idx < kTile * kEdgeCap; idx += blockDim.x) {
+      const int ll = idx / kEdgeCap, e = idx % kEdgeCap;
+      if (e >= s.n[ll]) continue;
+      const int ce = s.ec[ll][e];
+      const int cp = (e == 0) ? -(1 << 30) : s.ec[ll][e - 1];
+      const int jlo = max(cp + 1 - i0, 0), jhi = min(ce - i0, kTile);
+      for (int j = jlo; j <= jhi; ++j) s.first[ll][j] = e;
+    }
+    __syncthreads();
+    // D: accumulate
+    if (inside) {
+      if (!s.fallback[par]) {
+        const int ll = xaxis ? ty : tx;
+        const int j = xaxis ? tx : (kTile - 1 - ty);
+        const int i = i0 + j;
+        const int klo = s.klo[ll], n = s.n[ll];
+        const int e0 = max(s.first[ll][j] - 1, 0);
+        const int e1 = min(s.first[ll][j + 1] - 1, n - 2);
+        float gw = (WM == W_FP) ? pint[static_cast<size_t>(v) * g.Lmax + lbase + ll] : kbp;
+        if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
+        const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+        for (int e = e0; e <= e1; ++e) {
+          const EdgePos p1{s.ec[ll][e], s.ef[ll][e]};
+          const EdgePos p2{s.ec[ll][e + 1], s.ef[ll][e + 1]};
+          const float w = overlap_in_pixel(p1, p2, i) * gw;
+          float q[BT];
+          load_tile<BT>(sv + static_cast<size_t>(klo + e) * g.Bp, q);
+#pragma unroll
+          for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
+        }
+      } else {
+        pixel_one_view<BT, WM>(g, vg, v, edges, lines, pint, sino, r, c, b0, xcw, ycw, acc);
+      }
+    }
+    __syncthreads();
+  }
+  if (inside) {
+#pragma unroll
+    for (int b = 0; b < BT; ++b)
+      if (b0 + b < g.B)
+        out[(static_cast<size_t>(b0 + b) * g.H + r) * g.W + c] = acc[b];
+  }
+}
+
+}  // namespace body2d
+}  // namespace dd

@@ -1,0 +1,366 @@
+// Batched 2D projection of many instances of uneven size in one persistent
+// launch per phase.
+//
+// A batch is a list of instances, each with its own geometry (image size,
+// detector, view count and angles) and its own slot in one packed image
+// buffer and one packed sinogram buffer.  setup() builds every instance's
+// FP64 geometry tables once and uploads a device descriptor table; run() is
+// then two launches per op for the whole batch — a layout phase and a gather
+// phase — each a persistent grid that walks a static list of work items
+// (instance, view, bin chunk) or (instance, pixel tile), heaviest instances
+// first.  No counters, no atomics, no host work: a run is capturable and
+// replays bit-identically.
+//
+// The per-item arithmetic is the one of the one-call kernels
+// (dd2d_body.h), so a batched result equals the per-instance call exactly.
+// No counterpart in the reference (one launch per instance and op,
+// deeprecon/cuda/dis2d/FanArcDisCUDA2d.cu:758-1016).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "dd.h"
+#include "dd2d_body.h"
+#include "dd_common.h"
+#include "dd_profile.h"
+
+namespace dd {
+namespace {
+
+using namespace body2d;
+
+// Device descriptor of one instance.
+struct BatchInst {
+  Geo2 g;
+  ViewGeo* views;
+  double* edges;
+  float* wfp;
+  LineGeo* lines;
+  float* pint;
+  float* xrow;         // layout scratch: image rows, interleaved
+  float* xcol;         // layout scratch: image columns, bottom-up
+  float* sino;         // layout scratch: sinogram in walk order
+  long long img_off;   // floats, into the packed image buffer
+  long long sino_off;  // floats, into the packed sinogram buffer
+  int item0[4];        // first work item of each phase (see Phase)
+  int nitems[4];
+};
+
+enum Phase { kLayoutImage = 0, kGatherBins = 1, kLayoutSino = 2, kGatherPixels = 3 };
+
+constexpr int kBT = 1;  // one geometry per instance
+
+__device__ __forceinline__ int find_inst(const BatchInst* __restrict__ inst, int n,
+                                         int phase, int item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (inst[mid].item0[phase] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_batch_setup(
+    const BatchInst* __restrict__ inst, const float* const* __restrict__ angles,
+    int n_inst, int total) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    // setup items: one per (instance, view); item0[kLayoutImage] is unused here,
+    // so the per-instance view offsets are recomputed by a linear scan
+    int i = 0, base = 0;
+    while (i < n_inst - 1 && item >= base + inst[i].g.V) {
+      base += inst[i].g.V;
+      ++i;
+    }
+    const BatchInst& d = inst[i];
+    setup2d_view(d.g, angles[i], d.views, d.edges, d.wfp, d.lines, d.pint, item - base);
+    __syncthreads();
+  }
+}
+
+template <bool FOLD_W>
+__global__ __launch_bounds__(kThreads) void k_batch_layout(
+    const BatchInst* __restrict__ inst, int n_inst, int phase, int total,
+    const float* __restrict__ in) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(inst, n_inst, phase, item);
+    const BatchInst& d = inst[i];
+    const int local = item - d.item0[phase];
+    if (phase == kLayoutImage) {
+      // item = (image row, instance tile); threads stride over the row
+      const int h = local % d.g.H;
+      const int b0 = (local / d.g.H) * kBT;
+      for (int w = threadIdx.x; w < d.g.W; w += blockDim.x)
+        interleave_image_px<kBT>(d.g, in + d.img_off, d.xrow, d.xcol, h, w, b0);
+    } else {
+      // item = (view, instance tile); threads stride over the bins
+      const int v = local % d.g.V;
+      const int b0 = (local / d.g.V) * kBT;
+      for (int k = threadIdx.x; k < d.g.D; k += blockDim.x)
+        interleave_sino_bin<kBT, FOLD_W>(d.g, d.views, d.wfp, in + d.sino_off, d.sino, v, k, b0);
+    }
+  }
+}
+
+template <int WM>
+__global__ __launch_bounds__(kThreads) void k_batch_bins(
+    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(inst, n_inst, kGatherBins, item);
+    const BatchInst& d = inst[i];
+    const int local = item - d.item0[kGatherBins];
+    const int chunks = (d.g.D + kThreads - 1) / kThreads;
+    const int v = local % d.g.V;
+    const int rest = local / d.g.V;
+    const int k = (rest % chunks) * kThreads + threadIdx.x;
+    const int b0 = (rest / chunks) * kBT;
+    if (k < d.g.D)
+      bins_body<kBT, WM>(d.g, d.views, d.edges, d.wfp, d.lines, d.pint, d.xrow, d.xcol,
+                         out + d.sino_off, v, k, b0);
+  }
+}
+
+template <int WM>
+__global__ __launch_bounds__(kThreads) void k_batch_pixels(
+    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out,
+    int per_pixel) {
+  __shared__ PixelTileLds<kTile> lds;
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(inst, n_inst, kGatherPixels, item);
+    const BatchInst& d = inst[i];
+    const int local = item - d.item0[kGatherPixels];
+    const int tx = (d.g.W + kTile - 1) / kTile, ty = (d.g.H + kTile - 1) / kTile;
+    const int t = local % (tx * ty);
+    const int b0 = (local / (tx * ty)) * kBT;
+    if (per_pixel) {
+      const int c = (t % tx) * kTile + (threadIdx.x % kTile);
+      const int r = (t / tx) * kTile + (threadIdx.x / kTile);
+      if (c < d.g.W && r < d.g.H)
+        pixels_body<kBT, WM>(d.g, d.views, d.edges, d.lines, d.pint, d.sino, out + d.img_off,
+                             r, c, b0);
+    } else {
+      pixels_tile_body<kBT, WM, kTile>(d.g, d.views, d.edges, d.lines, d.pint, d.sino,
+                                       out + d.img_off, t % tx, t / tx, b0, lds);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dd
+
+using namespace dd;
+
+struct dd2d_batch {
+  std::vector<dd2d_batch_item> items;
+  std::vector<BatchInst> inst;   // host copy of the descriptors
+  std::vector<int> order;        // instance order (heaviest first)
+  size_t desc_off = 0, angles_off = 0, bytes = 0;
+  size_t total_views = 0;
+  int total[4] = {0, 0, 0, 0};
+  int grid = 0;                  // persistent workgroups (0 = automatic)
+  int variant = DD_ADJ_PER_PIXEL; // sinogram -> image kernel (per pixel or 16x16 tiles)
+  int cus = 0;
+  char* dev = nullptr;           // device memory given to setup()
+};
+
+namespace {
+
+size_t take(size_t* off, size_t bytes) {
+  const size_t at = *off;
+  *off = align_up(*off + bytes, static_cast<size_t>(256));
+  return at;
+}
+
+int grid_for(const dd2d_batch* b, int total) {
+  const int g = b->grid > 0 ? b->grid : std::max(1, b->cus) * 8;
+  return std::max(1, std::min(g, total));
+}
+
+}  // namespace
+
+extern "C" {
+
+dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
+  if (!items || n <= 0) return nullptr;
+  auto* b = new (std::nothrow) dd2d_batch();
+  if (!b) return nullptr;
+  b->items.assign(items, items + n);
+  for (int i = 0; i < n; ++i) {
+    const auto& it = items[i];
+    if (it.scan < DD_PARA || it.scan > DD_FAN_ARC || it.views <= 0 || it.batch <= 0 ||
+        it.geom.width <= 0 || it.geom.height <= 0 || it.geom.num_det <= 0 || !it.angles ||
+        it.image_offset < 0 || it.sino_offset < 0) {
+      delete b;
+      return nullptr;
+    }
+  }
+  // heaviest instances first, so the static item lists balance
+  b->order.resize(n);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+    const auto& a = items[x];
+    const auto& c = items[y];
+    return 1.0 * a.views * a.geom.width * a.geom.height * a.batch >
+           1.0 * c.views * c.geom.width * c.geom.height * c.batch;
+  });
+  size_t off = 0;
+  b->desc_off = take(&off, sizeof(BatchInst) * n);
+  b->angles_off = take(&off, sizeof(float*) * n);
+  b->inst.resize(n);
+  int acc[4] = {0, 0, 0, 0};
+  for (int j = 0; j < n; ++j) {
+    const auto& it = items[b->order[j]];
+    BatchInst d{};
+    Geo2& g = d.g;
+    g.W = it.geom.width;
+    g.H = it.geom.height;
+    g.D = it.geom.num_det;
+    g.V = it.views;
+    g.B = it.batch;
+    g.BT = kBT;
+    g.Bp = it.batch;
+    g.Lmax = std::max(g.W, g.H);
+    g.scan = it.scan;
+    g.dx = it.geom.pix_x;
+    g.dy = it.geom.pix_y;
+    g.det = it.geom.det;
+    g.sid = it.geom.iso_source;
+    g.sdd = it.geom.source_detector;
+    g.shx = it.geom.shift_x;
+    g.shy = it.geom.shift_y;
+    g.bin = it.geom.bin_shift;
+    // geometry tables and layout scratch (offsets now, pointers at setup)
+    d.views = reinterpret_cast<ViewGeo*>(take(&off, sizeof(ViewGeo) * g.V));
+    d.edges = reinterpret_cast<double*>(take(&off, sizeof(double) * g.V * (g.D + 1)));
+    d.wfp = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.D));
+    d.lines = reinterpret_cast<LineGeo*>(take(&off, sizeof(LineGeo) * g.V * g.Lmax));
+    d.pint = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.Lmax));
+    const size_t img = static_cast<size_t>(g.W) * g.H * g.Bp;
+    d.xrow = reinterpret_cast<float*>(take(&off, sizeof(float) * img));
+    d.xcol = reinterpret_cast<float*>(take(&off, sizeof(float) * img));
+    d.sino = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.D * g.Bp));
+    d.img_off = it.image_offset;
+    d.sino_off = it.sino_offset;
+    const int tiles = it.batch;  // kBT == 1
+    d.nitems[kLayoutImage] = g.H * tiles;
+    d.nitems[kGatherBins] = g.V * ((g.D + kThreads - 1) / kThreads) * tiles;
+    d.nitems[kLayoutSino] = g.V * tiles;
+    d.nitems[kGatherPixels] = ((g.W + kTile - 1) / kTile) * ((g.H + kTile - 1) / kTile) * tiles;
+    for (int ph = 0; ph < 4; ++ph) {
+      d.item0[ph] = acc[ph];
+      acc[ph] += d.nitems[ph];
+    }
+    b->total_views += g.V;
+    b->inst[j] = d;
+  }
+  for (int ph = 0; ph < 4; ++ph) b->total[ph] = acc[ph];
+  b->bytes = off;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    b->cus = 256;
+  return b;
+}
+
+size_t dd2d_batch_device_bytes(const dd2d_batch* b) { return b ? b->bytes : 0; }
+
+void dd2d_batch_set_grid(dd2d_batch* b, int workgroups) {
+  if (b) b->grid = workgroups > 0 ? workgroups : 0;
+}
+
+void dd2d_batch_set_variant(dd2d_batch* b, int variant) {
+  if (b) b->variant = (variant == DD_ADJ_TILE16) ? DD_ADJ_TILE16 : DD_ADJ_PER_PIXEL;
+}
+
+dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* stream) {
+  if (!b || !device_mem || bytes < b->bytes) return DD_NO_WORKSPACE;
+  b->dev = static_cast<char*>(device_mem);
+  auto st = static_cast<hipStream_t>(stream);
+  const int n = static_cast<int>(b->inst.size());
+  std::vector<BatchInst> host(b->inst);
+  std::vector<const float*> angles(n);
+  auto rebase = [&](auto* p) {
+    return reinterpret_cast<decltype(p)>(b->dev + reinterpret_cast<size_t>(p));
+  };
+  for (int j = 0; j < n; ++j) {
+    BatchInst& d = host[j];
+    d.views = rebase(d.views);
+    d.edges = rebase(d.edges);
+    d.wfp = rebase(d.wfp);
+    d.lines = rebase(d.lines);
+    d.pint = rebase(d.pint);
+    d.xrow = rebase(d.xrow);
+    d.xcol = rebase(d.xcol);
+    d.sino = rebase(d.sino);
+    angles[j] = b->items[b->order[j]].angles;
+  }
+  b->inst = host;
+  if (hipMemcpyAsync(b->dev + b->desc_off, host.data(), sizeof(BatchInst) * n,
+                     hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(b->dev + b->angles_off, angles.data(), sizeof(float*) * n,
+                     hipMemcpyHostToDevice, st) != hipSuccess)
+    return DD_LAUNCH_FAIL;
+  const int total = static_cast<int>(b->total_views);
+  hipLaunchKernelGGL(k_batch_setup, dim3(grid_for(b, total)), dim3(kThreads), 0, st,
+                     reinterpret_cast<const BatchInst*>(b->dev + b->desc_off),
+                     reinterpret_cast<const float* const*>(b->dev + b->angles_off), n, total);
+  if (hipGetLastError() != hipSuccess) return DD_LAUNCH_FAIL;
+  // the host staging vectors die with this call: wait for the uploads
+  return hipStreamSynchronize(st) == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+}
+
+dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float* output,
+                         void* stream) {
+  if (!b || !b->dev || !input || !output) return DD_BAD_ARG;
+  if (op < DD_PROJ || op > DD_BPROJ_TW) return DD_BAD_ARG;
+  for (const auto& it : b->items)
+    if (it.scan == DD_PARA && op >= DD_BPROJ_W) return DD_UNSUPPORTED;
+  auto st = static_cast<hipStream_t>(stream);
+  const int n = static_cast<int>(b->inst.size());
+  const auto* inst = reinterpret_cast<const BatchInst*>(b->dev + b->desc_off);
+  const bool sino_out = (op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW);
+  if (sino_out) {
+    hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutImage])),
+                       dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutImage),
+                       b->total[kLayoutImage], input);
+    const int tot = b->total[kGatherBins];
+    ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
+    if (op == DD_PROJ)
+      hipLaunchKernelGGL((k_batch_bins<W_FP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
+                         inst, n, tot, output);
+    else if (op == DD_BPROJ_T)
+      hipLaunchKernelGGL((k_batch_bins<W_BP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
+                         inst, n, tot, output);
+    else
+      hipLaunchKernelGGL((k_batch_bins<W_BPW>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
+                         inst, n, tot, output);
+  } else {
+    if (op == DD_PROJ_T)
+      hipLaunchKernelGGL((k_batch_layout<true>), dim3(grid_for(b, b->total[kLayoutSino])),
+                         dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutSino),
+                         b->total[kLayoutSino], input);
+    else
+      hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutSino])),
+                         dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutSino),
+                         b->total[kLayoutSino], input);
+    const int tot = b->total[kGatherPixels];
+    ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
+    if (op == DD_PROJ_T)
+      hipLaunchKernelGGL((k_batch_pixels<W_FP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
+                         inst, n, tot, output, b->variant != DD_ADJ_TILE16);
+    else if (op == DD_BPROJ)
+      hipLaunchKernelGGL((k_batch_pixels<W_BP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
+                         inst, n, tot, output, b->variant != DD_ADJ_TILE16);
+    else
+      hipLaunchKernelGGL((k_batch_pixels<W_BPW>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
+                         inst, n, tot, output, b->variant != DD_ADJ_TILE16);
+  }
+  return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+}
+
+void dd2d_batch_destroy(dd2d_batch* b) { delete b; }
+
+}  // extern "C"

@@ -298,8 +298,10 @@ torch::Tensor op3d(int scan, int op, const torch::Tensor& input,
 class Plan2d {
  public:
   Plan2d(int scan, const torch::Tensor& options, const torch::Tensor& angles,
-         int64_t batch)
-      : scan_(scan), batch_(batch) {
+         int64_t batch, int64_t tile, int64_t variant)
+      : scan_(scan), batch_(batch), tile_(static_cast<int>(tile)),
+        variant_(static_cast<int>(variant)) {
+    TORCH_CHECK(variant >= -1 && variant <= DD_ADJ_TILE8, "Plan2d: unknown kernel variant");
     TORCH_CHECK(scan >= DD_PARA && scan <= DD_FAN_ARC, "Plan2d: 2D scan expected");
     TORCH_CHECK(angles.is_cuda() && angles.scalar_type() == torch::kFloat32 &&
                     angles.dim() == 1,
@@ -319,7 +321,7 @@ class Plan2d {
     for (int op = DD_PROJ; op <= DD_BPROJ_TW; ++op) {
       if (scan == DD_PARA && op >= DD_BPROJ_W) break;
       ws_bytes_[op] = dd2d_planned_workspace_size(scan, op, &g_, static_cast<int>(batch),
-                                                  static_cast<int>(views_));
+                                                  static_cast<int>(views_), tile_);
       // allocated up front so no allocation happens inside a graph capture
       ws_[op] = workspace(ws_bytes_[op], angles_);
     }
@@ -349,23 +351,108 @@ class Plan2d {
                                   input.options());
     DD_CHECK(dd2d_run_planned(scan_, static_cast<int>(op), &g_, geo_.data_ptr(),
                               input.data_ptr<float>(), out.data_ptr<float>(),
-                              static_cast<int>(batch_), static_cast<int>(views_),
-                              ws_[op].data_ptr(), ws_bytes_[op], cur_stream(input)),
+                              static_cast<int>(batch_), static_cast<int>(views_), tile_,
+                              variant_, ws_[op].data_ptr(), ws_bytes_[op], cur_stream(input)),
              "Plan2d run");
     return out;
   }
 
   int64_t batch() const { return batch_; }
+  int64_t tile() const { return tile_; }
+  int64_t variant() const { return variant_; }
   int64_t views() const { return views_; }
 
  private:
   int scan_;
   int64_t batch_;
+  int tile_ = 0;
+  int variant_ = -1;
   int64_t views_ = 0;
   dd_geom2d g_{};
   torch::Tensor angles_, geo_;
   torch::Tensor ws_[6];
   size_t ws_bytes_[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// ---- batched uneven instances ----------------------------------------------
+// Instances of different image size, detector and view count, packed into
+// one flat image buffer and one flat sinogram buffer; every op is two
+// persistent launches for the whole batch (ddbatch.hip).
+class PlanBatch2d {
+ public:
+  PlanBatch2d(int scan, const std::vector<torch::Tensor>& options,
+              const std::vector<torch::Tensor>& angles, int64_t grid) {
+    TORCH_CHECK(scan >= DD_PARA && scan <= DD_FAN_ARC, "PlanBatch2d: 2D scan expected");
+    TORCH_CHECK(!options.empty() && options.size() == angles.size(),
+                "PlanBatch2d: one options and one angles tensor per instance");
+    std::vector<dd2d_batch_item> items(options.size());
+    int64_t img = 0, sino = 0;
+    for (size_t i = 0; i < options.size(); ++i) {
+      TORCH_CHECK(angles[i].is_cuda() && angles[i].scalar_type() == torch::kFloat32 &&
+                      angles[i].dim() == 1 && angles[i].is_contiguous(),
+                  "PlanBatch2d: angles must be contiguous 1D float32 device tensors");
+      angles_.push_back(angles[i]);
+      auto& it = items[i];
+      it.scan = scan;
+      it.geom = decode2d(options[i], "PlanBatch2d");
+      it.views = static_cast<int>(angles[i].size(0));
+      it.batch = 1;
+      it.angles = angles[i].data_ptr<float>();
+      it.image_offset = img;
+      it.sino_offset = sino;
+      image_off_.push_back(img);
+      sino_off_.push_back(sino);
+      img += static_cast<int64_t>(it.geom.width) * it.geom.height;
+      sino += static_cast<int64_t>(it.views) * it.geom.num_det;
+      shapes_.push_back({it.geom.height, it.geom.width, it.views, it.geom.num_det});
+    }
+    image_size_ = img;
+    sino_size_ = sino;
+    b_ = dd2d_batch_create(items.data(), static_cast<int>(items.size()));
+    TORCH_CHECK(b_ != nullptr, "PlanBatch2d: invalid instance description");
+    if (grid > 0) dd2d_batch_set_grid(b_, static_cast<int>(grid));
+    const auto& a0 = angles_.front();
+    c10::hip::HIPGuard guard(a0.device().index());
+    mem_ = workspace(dd2d_batch_device_bytes(b_), a0);
+    DD_CHECK(dd2d_batch_setup(b_, mem_.data_ptr(), dd2d_batch_device_bytes(b_), cur_stream(a0)),
+             "PlanBatch2d setup");
+  }
+  ~PlanBatch2d() { dd2d_batch_destroy(b_); }
+  PlanBatch2d(const PlanBatch2d&) = delete;
+  PlanBatch2d& operator=(const PlanBatch2d&) = delete;
+
+  // op on the packed input (flat float32 device tensor); returns a new
+  // packed output.
+  torch::Tensor run(int64_t op, const torch::Tensor& input) {
+    TORCH_CHECK(input.is_cuda() && input.scalar_type() == torch::kFloat32 &&
+                    input.is_contiguous(),
+                "PlanBatch2d: packed input must be a contiguous float32 device tensor");
+    const bool to_sino = (op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW);
+    TORCH_CHECK(input.numel() == (to_sino ? image_size_ : sino_size_),
+                "PlanBatch2d: packed input has ", input.numel(), " values, expected ",
+                to_sino ? image_size_ : sino_size_);
+    c10::hip::HIPGuard guard(input.device().index());
+    auto out = torch::empty({to_sino ? sino_size_ : image_size_}, input.options());
+    DD_CHECK(dd2d_batch_run(b_, static_cast<int>(op), input.data_ptr<float>(),
+                            out.data_ptr<float>(), cur_stream(input)),
+             "PlanBatch2d run");
+    return out;
+  }
+  void set_grid(int64_t g) { dd2d_batch_set_grid(b_, static_cast<int>(g)); }
+  void set_variant(int64_t v) { dd2d_batch_set_variant(b_, static_cast<int>(v)); }
+  std::vector<int64_t> image_offsets() const { return image_off_; }
+  std::vector<int64_t> sino_offsets() const { return sino_off_; }
+  std::vector<std::vector<int64_t>> shapes() const { return shapes_; }
+  int64_t image_size() const { return image_size_; }
+  int64_t sino_size() const { return sino_size_; }
+
+ private:
+  dd2d_batch* b_ = nullptr;
+  torch::Tensor mem_;
+  std::vector<torch::Tensor> angles_;
+  std::vector<int64_t> image_off_, sino_off_;
+  std::vector<std::vector<int64_t>> shapes_;
+  int64_t image_size_ = 0, sino_size_ = 0;
 };
 
 }  // namespace
@@ -474,12 +561,28 @@ PYBIND11_MODULE(radontorch, m) {
   REG(GetSysMatrixPara2d);
 #undef REG
   pybind11::class_<Plan2d>(m, "Plan2d")
-      .def(pybind11::init<int, const torch::Tensor&, const torch::Tensor&, int64_t>(),
+      .def(pybind11::init<int, const torch::Tensor&, const torch::Tensor&, int64_t, int64_t,
+                          int64_t>(),
            pybind11::arg("scan"), pybind11::arg("options"), pybind11::arg("angles"),
-           pybind11::arg("batch"))
+           pybind11::arg("batch"), pybind11::arg("tile") = 0, pybind11::arg("variant") = -1)
       .def("run", &Plan2d::run)
+      .def_property_readonly("tile", &Plan2d::tile)
+      .def_property_readonly("variant", &Plan2d::variant)
       .def_property_readonly("batch", &Plan2d::batch)
       .def_property_readonly("views", &Plan2d::views);
+  pybind11::class_<PlanBatch2d>(m, "PlanBatch2d")
+      .def(pybind11::init<int, const std::vector<torch::Tensor>&,
+                          const std::vector<torch::Tensor>&, int64_t>(),
+           pybind11::arg("scan"), pybind11::arg("options"), pybind11::arg("angles"),
+           pybind11::arg("grid") = 0)
+      .def("run", &PlanBatch2d::run)
+      .def("set_grid", &PlanBatch2d::set_grid)
+      .def("set_variant", &PlanBatch2d::set_variant)
+      .def_property_readonly("image_offsets", &PlanBatch2d::image_offsets)
+      .def_property_readonly("sino_offsets", &PlanBatch2d::sino_offsets)
+      .def_property_readonly("shapes", &PlanBatch2d::shapes)
+      .def_property_readonly("image_size", &PlanBatch2d::image_size)
+      .def_property_readonly("sino_size", &PlanBatch2d::sino_size);
   m.def("profile_enable", [](bool on) { dd_profile_enable(on ? 1 : 0); });
   m.def("profile_read", [](int cls) {
     double ms = 0.0;
@@ -488,6 +591,10 @@ PYBIND11_MODULE(radontorch, m) {
     return pybind11::make_tuple(ms, n);
   });
   m.def("set_instance_tile", [](int t) { dd2d_set_instance_tile(t); });
+  m.def("set_adjoint_variant", [](int v) { dd2d_set_adjoint_variant(v); });
+  m.attr("ADJ_PER_PIXEL") = static_cast<int>(DD_ADJ_PER_PIXEL);
+  m.attr("ADJ_TILE16") = static_cast<int>(DD_ADJ_TILE16);
+  m.attr("ADJ_TILE8") = static_cast<int>(DD_ADJ_TILE8);
   m.attr("PROJ") = static_cast<int>(DD_PROJ);
   m.attr("PROJ_T") = static_cast<int>(DD_PROJ_T);
   m.attr("BPROJ") = static_cast<int>(DD_BPROJ);
