@@ -14,11 +14,14 @@
 //                        law, per-line x and z maps;
 //   k_interleave_volume  volume -> [line][slice][pixel][b] copies (rows and
 //                        bottom-up columns) for image -> sinogram ops;
-//   k_bins3d             image -> sinogram: one thread per detector cell,
-//                        gathering over the lines its ray band meets in z;
-//   k_interleave_sino3d  sinogram -> [view][row][col][b] (x FP weight);
-//   k_voxels3d           sinogram -> image: one thread per voxel, gathering
-//                        over views.
+//   k_bins3d             image -> sinogram: one thread per detector column and
+//                        chunk of rows, gathering over the lines its ray band
+//                        meets in z;
+//   k_prefix_sino3d      sinogram -> per-column running sums over detector
+//                        rows, [view][row][col][b] (x FP weight);
+//   k_voxels3d           sinogram -> image: one thread per pixel and chunk of
+//                        slices, gathering over views; the row/slice overlap
+//                        sums come from differences of the running sums.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,9 +35,9 @@ namespace dd {
 namespace {
 
 constexpr int kThreads3 = 256;
-constexpr int kColsPerBlock = 64;  // k_bins3d: 64 columns x 4 rows
+constexpr int kColsPerBlock = 64;  // k_bins3d: 64 columns x 4 row chunks
 constexpr int kRowsPerBlock = 4;
-constexpr int kTile3 = 16;         // k_voxels3d: 16 x 16 voxels of one slice
+constexpr int kTile3 = 16;         // k_voxels3d: 16 x 16 pixels x one slice chunk
 
 enum WMode3 { W3_FP = 0, W3_BP = 1, W3_BPW = 2 };
 
@@ -51,6 +54,7 @@ struct ViewGeo3 {
   double zdet0, rowp;  // detector row edge r at zdet0 + r * rowp
   double sid_eff;      // source-isocentre distance of this view
   float sxf, syf, kbp; // float copies; BackProj constant
+  float zlo, zhi;      // world z range the view's rays can reach in the volume
   int xaxis, flip;
 };
 
@@ -138,8 +142,8 @@ __global__ __launch_bounds__(kThreads3) void k_setup3d(
   const double vcol = static_cast<double>(g.sid) / g.sdd * g.col;
   const double vcsh = static_cast<double>(g.sid) / g.sdd * g.csh;
   const bool flat = (g.scan == DD_CONE_FLAT);
-  if (threadIdx.x == 0) {
-    ViewGeo3 vg;
+  ViewGeo3 vg;
+  {
     vg.sx = sx;
     vg.sy = sy;
     vg.sz = sz;
@@ -157,8 +161,8 @@ __global__ __launch_bounds__(kThreads3) void k_setup3d(
     vg.kbp = static_cast<float>(flat ? vcol : sid * colp);
     vg.xaxis = xaxis;
     vg.flip = flip;
-    views[v] = vg;
   }
+  double mag_lo = 1e300, mag_hi = -1e300, m_lo = 1e300, m_hi = -1e300;
   // column edges on the common axis, walk order (ConeFlatDisCUDA3d.cu:78-91,
   // ConeArcDisCUDA3d.cu:82-93)
   for (int k = threadIdx.x; k <= C; k += blockDim.x) {
@@ -186,7 +190,10 @@ __global__ __launch_bounds__(kThreads3) void k_setup3d(
       dcx = sin(gc) * sdd + sx;
       dcy = -cos(gc) * sdd + sy;
     }
-    mag[static_cast<size_t>(v) * C + k] = xaxis ? sy / (sy - dcy) : sx / (sx - dcx);
+    const double mk = xaxis ? sy / (sy - dcy) : sx / (sx - dcx);
+    mag[static_cast<size_t>(v) * C + k] = mk;
+    mag_lo = fmin(mag_lo, mk);
+    mag_hi = fmax(mag_hi, mk);
   }
   // per line: pixel edges on the axis and slice edges on the axis plane
   const int L = xaxis ? g.H : g.W;
@@ -215,6 +222,34 @@ __global__ __launch_bounds__(kThreads3) void k_setup3d(
     lg.p = static_cast<float>(p);
     lg.pzf = static_cast<float>(lg.pz);
     lines[static_cast<size_t>(v) * g.Lmax + l] = lg;
+    m_lo = fmin(m_lo, m);
+    m_hi = fmax(m_hi, m);
+  }
+  // World z range of the view: a slice at z meets row band zdet on line l
+  // and column k only if m_l (z - sz) lies within mag_k (zdet - sz), so z - sz
+  // is bounded by (mag_k / m_l) times the detector's z extent, over all k, l.
+  __shared__ double red[4][kThreads3];
+  red[0][threadIdx.x] = mag_lo;
+  red[1][threadIdx.x] = -mag_hi;
+  red[2][threadIdx.x] = m_lo;
+  red[3][threadIdx.x] = -m_hi;
+  __syncthreads();
+  for (int w = kThreads3 / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int q = 0; q < 4; ++q)
+        red[q][threadIdx.x] = fmin(red[q][threadIdx.x], red[q][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rho_lo = red[0][0] / -red[3][0], rho_hi = -red[1][0] / red[2][0];
+    const double d0 = vg.zdet0 - sz, d1 = fma(static_cast<double>(g.R), vg.rowp, vg.zdet0) - sz;
+    const double c0 = rho_lo * d0, c1 = rho_lo * d1, c2 = rho_hi * d0, c3 = rho_hi * d1;
+    const double lo = fmin(fmin(c0, c1), fmin(c2, c3)), hi = fmax(fmax(c0, c1), fmax(c2, c3));
+    // one slice of margin on each side, plus a relative guard
+    const double pad = g.dz + 1e-6 * (fabs(lo) + fabs(hi) + fabs(sz));
+    vg.zlo = static_cast<float>(sz + lo - pad);
+    vg.zhi = static_cast<float>(sz + hi + pad);
+    views[v] = vg;
   }
 }
 
@@ -242,34 +277,58 @@ __global__ __launch_bounds__(kThreads3) void k_interleave_volume(
   store_tile<BT>(xcol + ((static_cast<size_t>(w) * g.Dz + s) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
 }
 
-// p [B][V][R][C] -> s [V][R][C][Bp], columns in walk order, optionally times
-// the FP weight of the cell.
+// p [B][V][R][C] -> Q [V][R+1][C][Bp]: per view and column, the running sum
+// over detector rows (columns in walk order, optionally times the FP weight
+// of each cell), Q[0] = 0.  Accumulated in FP64 and stored rounded, so every
+// entry carries one rounding and no drift.
 template <int BT, bool FOLD_W>
-__global__ __launch_bounds__(kThreads3) void k_interleave_sino3d(
+__global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const float* __restrict__ p,
-    float* __restrict__ out) {
+    float* __restrict__ q) {
   const int nkc = (g.C + kThreads3 - 1) / kThreads3;
   const int k = (blockIdx.x % nkc) * kThreads3 + threadIdx.x;
-  const int r = (blockIdx.x / nkc) % g.R;
-  const int v = blockIdx.x / (nkc * g.R);
+  const int v = blockIdx.x / nkc;
   const int b0 = blockIdx.z * BT;
   if (k >= g.C) return;
   const ViewGeo3 vg = views[v];
   const int kk = vg.flip ? g.C - 1 - k : k;
-  float w = 1.0f;
+  double e1 = 0.0, e2 = 0.0, m = 0.0;
   if (FOLD_W) {
-    const double m = mag[static_cast<size_t>(v) * g.C + k];
     const double* ce = cedge + static_cast<size_t>(v) * (g.C + 1);
-    w = fp_weight3(g, vg, ce[k], ce[k + 1], row_z(vg, m, r), row_z(vg, m, r + 1));
+    e1 = ce[k];
+    e2 = ce[k + 1];
+    m = mag[static_cast<size_t>(v) * g.C + k];
   }
+  const size_t rstride = static_cast<size_t>(g.C) * g.Bp;
+  float* __restrict__ qv = q + static_cast<size_t>(v) * (g.R + 1) * rstride +
+                           static_cast<size_t>(k) * g.Bp + b0;
+  double acc[BT];
   float val[BT];
 #pragma unroll
-  for (int b = 0; b < BT; ++b)
-    val[b] = (b0 + b < g.B)
-                 ? p[((static_cast<size_t>(b0 + b) * g.V + v) * g.R + r) * g.C + kk] * w
-                 : 0.0f;
-  store_tile<BT>(out + ((static_cast<size_t>(v) * g.R + r) * g.C + k) * g.Bp + b0, val);
+  for (int b = 0; b < BT; ++b) {
+    acc[b] = 0.0;
+    val[b] = 0.0f;
+  }
+  store_tile<BT>(qv, val);
+  double zlo = FOLD_W ? row_z(vg, m, 0) : 0.0;
+  for (int r = 0; r < g.R; ++r) {
+    float w = 1.0f;
+    if (FOLD_W) {
+      const double zhi = row_z(vg, m, r + 1);
+      w = fp_weight3(g, vg, e1, e2, zlo, zhi);
+      zlo = zhi;
+    }
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const float x = (b0 + b < g.B)
+                          ? p[((static_cast<size_t>(b0 + b) * g.V + v) * g.R + r) * g.C + kk]
+                          : 0.0f;
+      acc[b] += static_cast<double>(x * w);
+      val[b] = static_cast<float>(acc[b]);
+    }
+    store_tile<BT>(qv + (r + 1) * rstride, val);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -305,18 +364,59 @@ __device__ __forceinline__ void line_range(const LineGeo3* __restrict__ lv,
   *last = hi;
 }
 
+// Detector row edges of column k on the slice grid of line l.  Row edge r
+// sits at z = m (zdet0 + r rowp - sz) + sz on the axis plane, so in slice
+// units t_r = A + r B, affine in r.  The forward kernel walks these edges;
+// the adjoint inverts the same law (u = (t - A) / B).
+struct ZAffine {
+  double A, B;
+};
+
+__device__ __forceinline__ ZAffine z_affine(const ViewGeo3& vg, const LineGeo3& lg,
+                                            double m) {
+  const double u = fma(m, vg.zdet0 - vg.sz, vg.sz);
+  return {(u - lg.z0) * lg.inv_pz, (m * vg.rowp) * lg.inv_pz};
+}
+
+__device__ __forceinline__ EdgePos row_pos(const ZAffine& z, int r, int Dz) {
+  return edge_pos_t(fma(static_cast<double>(r), z.B, z.A), Dz);
+}
+
+// Rows (forward) and slices (adjoint) one thread carries in registers.
+#ifndef DD3D_FWD_ROWS
+#define DD3D_FWD_ROWS 4
+#endif
+#ifndef DD3D_ADJ_SLICES
+#define DD3D_ADJ_SLICES 16
+#endif
+template <int BT>
+constexpr int rows3() {
+  return BT >= 4 ? (DD3D_FWD_ROWS > 1 ? DD3D_FWD_ROWS / 2 : 1) : DD3D_FWD_ROWS;
+}
+template <int BT>
+constexpr int slices3() {
+  return BT >= 4 ? DD3D_ADJ_SLICES / 2 : DD3D_ADJ_SLICES;
+}
+
+// image -> sinogram: one thread per (view, column, chunk of S rows).  Per
+// line the column's x footprint and the z law are evaluated once and shared
+// by the S rows; each row then walks its slices.
 template <int BT, int WM>
 __global__ __launch_bounds__(kThreads3) void k_bins3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
     const float* __restrict__ xrow, const float* __restrict__ xcol,
     float* __restrict__ out) {
+  constexpr int S = rows3<BT>();
   const int v = blockIdx.x;
   const int k = blockIdx.y * kColsPerBlock + (threadIdx.x % kColsPerBlock);
-  const int nrc = (g.R + kRowsPerBlock - 1) / kRowsPerBlock;
-  const int r = (blockIdx.z % nrc) * kRowsPerBlock + threadIdx.x / kColsPerBlock;
-  const int b0 = (blockIdx.z / nrc) * BT;
-  if (k >= g.C || r >= g.R) return;
+  const int nch = (g.R + S - 1) / S;
+  const int ncb = (nch + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int ch = (blockIdx.z % ncb) * kRowsPerBlock + threadIdx.x / kColsPerBlock;
+  const int b0 = (blockIdx.z / ncb) * BT;
+  const int r0 = ch * S;
+  if (k >= g.C || r0 >= g.R) return;
+  const int nr = min(S, g.R - r0);
   const ViewGeo3 vg = views[v];
   const int xaxis = vg.xaxis;
   const int L = xaxis ? g.H : g.W;
@@ -325,66 +425,90 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
   const double* ce = cedge + static_cast<size_t>(v) * (g.C + 1);
   const double e1 = ce[k], e2 = ce[k + 1];
   const double m = mag[static_cast<size_t>(v) * g.C + k];
-  const double zlo = row_z(vg, m, r), zhi = row_z(vg, m, r + 1);
   const LineGeo3* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
-  float acc[BT];
+  float acc[S][BT];
 #pragma unroll
-  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int b = 0; b < BT; ++b) acc[j][b] = 0.0f;
   int lfirst, llast;
-  line_range(lv, L, g.Dz, zlo, zhi, &lfirst, &llast);
+  line_range(lv, L, g.Dz, row_z(vg, m, r0), row_z(vg, m, r0 + nr), &lfirst, &llast);
+  // one line of margin: the range test and the slice walk round differently
+  lfirst = max(lfirst - 1, 0);
+  llast = min(llast + 1, L - 1);
   for (int l = lfirst; l <= llast; ++l) {
     const LineGeo3 lg = lv[l];
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
-    const int cs = max(p1.c, 0), ce2 = min(p2.c, N - 1);
-    if (cs > ce2) continue;
-    const EdgePos q1 = edge_pos(zlo, lg.z0, lg.inv_pz, g.Dz);
-    const EdgePos q2 = edge_pos(zhi, lg.z0, lg.inv_pz, g.Dz);
-    const int ss = max(q1.c, 0), se = min(q2.c, g.Dz - 1);
-    if (ss > se) continue;
+    const int cs = max(p1.c, 0), cl = min(p2.c, N - 1);
+    if (cs > cl) continue;
+    const ZAffine z = z_affine(vg, lg, m);
+    EdgePos qa = row_pos(z, r0, g.Dz);
+    const EdgePos qend = row_pos(z, r0 + nr, g.Dz);
+    if (qend.c < 0 || qa.c >= g.Dz) continue;  // the chunk's rows miss the slab
     const float wl = (WM == W3_FP) ? lg.p * lg.pzf : vg.kbp;
     const float* __restrict__ line = src + static_cast<size_t>(l) * g.Dz * N * g.Bp;
-    for (int sl = ss; sl <= se; ++sl) {
-      const float wz = overlap_in_pixel(q1, q2, sl) * wl;
-      const float* __restrict__ slab = line + static_cast<size_t>(sl) * N * g.Bp;
-      for (int c = cs; c <= ce2; ++c) {
-        float w = overlap_in_pixel(p1, p2, c) * wz;
-        if constexpr (WM == W3_BPW) {
-          float xc, yc;
-          pixel_centre3(g, xaxis, l, c, &xc, &yc);
-          w *= fbp_weight3(g, vg, xc, yc);
-        }
-        float xv[BT];
-        load_tile<BT>(slab + static_cast<size_t>(c) * g.Bp, xv);
 #pragma unroll
-        for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
+    for (int j = 0; j < S; ++j) {
+      if (j < nr) {
+        const EdgePos qb = (j + 1 == nr) ? qend : row_pos(z, r0 + j + 1, g.Dz);
+        const int ss = max(qa.c, 0), se = min(qb.c, g.Dz - 1);
+        for (int sl = ss; sl <= se; ++sl) {
+          const float wz = overlap_in_pixel(qa, qb, sl) * wl;
+          const float* __restrict__ slab = line + static_cast<size_t>(sl) * N * g.Bp;
+          for (int c = cs; c <= cl; ++c) {
+            float w = overlap_in_pixel(p1, p2, c) * wz;
+            if constexpr (WM == W3_BPW) {
+              float xc, yc;
+              pixel_centre3(g, xaxis, l, c, &xc, &yc);
+              w *= fbp_weight3(g, vg, xc, yc);
+            }
+            float xv[BT];
+            load_tile<BT>(slab + static_cast<size_t>(c) * g.Bp, xv);
+#pragma unroll
+            for (int b = 0; b < BT; ++b) acc[j][b] = fmaf(w, xv[b], acc[j][b]);
+          }
+        }
+        qa = qb;
       }
     }
   }
   const int kk = vg.flip ? g.C - 1 - k : k;
-  const float wk = (WM == W3_FP) ? fp_weight3(g, vg, e1, e2, zlo, zhi) : 1.0f;
 #pragma unroll
-  for (int b = 0; b < BT; ++b)
-    if (b0 + b < g.B)
-      out[((static_cast<size_t>(b0 + b) * g.V + v) * g.R + r) * g.C + kk] = acc[b] * wk;
+  for (int j = 0; j < S; ++j) {
+    if (j < nr) {
+      const int r = r0 + j;
+      const float wk = (WM == W3_FP)
+                           ? fp_weight3(g, vg, e1, e2, row_z(vg, m, r), row_z(vg, m, r + 1))
+                           : 1.0f;
+#pragma unroll
+      for (int b = 0; b < BT; ++b)
+        if (b0 + b < g.B)
+          out[((static_cast<size_t>(b0 + b) * g.V + v) * g.R + r) * g.C + kk] = acc[j][b] * wk;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
-// sinogram -> image: one thread per voxel, all views
+// sinogram -> image: one thread per (pixel, chunk of S slices), all views
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int guess_col(const Geo3& g, const ViewGeo3& vg, double a) {
-  double kf;
+// First-column guess in float; callers fix it exactly with edge_pos.
+__device__ __forceinline__ int guess_col_fast(const Geo3& g, const ViewGeo3& vg, double a) {
+  float kf;
   if (g.scan == DD_CONE_FLAT) {
-    const double R = g.sid;
-    const double u = vg.xaxis ? R * a * vg.c / (R + vg.s * a) : R * a * vg.s / (R - vg.c * a);
-    kf = g.C * 0.5 + (vg.flip ? -(u - vg.cshp) : (u - vg.cshp)) / vg.colp;
+    const float R = g.sid, af = static_cast<float>(a);
+    const float sf = static_cast<float>(vg.s), cf = static_cast<float>(vg.c);
+    const float u = vg.xaxis ? R * af * cf / (R + sf * af) : R * af * sf / (R - cf * af);
+    const float d = u - static_cast<float>(vg.cshp);
+    kf = g.C * 0.5f + (vg.flip ? -d : d) / static_cast<float>(vg.colp);
   } else {
-    double gam = vg.xaxis ? atan2(a - vg.sx, vg.sy) : atan2(vg.sx, a - vg.sy);
-    double d = gam - vg.theta_ref;
-    d -= M_PI * rint(d / M_PI);
-    kf = g.C * 0.5 + (vg.flip ? -d : d) / vg.colp;
+    const float gam = vg.xaxis ? atan2f(static_cast<float>(a - vg.sx), static_cast<float>(vg.sy))
+                               : atan2f(static_cast<float>(vg.sx), static_cast<float>(a - vg.sy));
+    float d = gam - static_cast<float>(vg.theta_ref);
+    d -= 3.14159265f * rintf(d * 0.318309886f);
+    kf = g.C * 0.5f + (vg.flip ? -d : d) / static_cast<float>(vg.colp);
   }
-  kf = fmin(fmax(kf, 0.0), g.C - 1.0);
+  kf = fminf(fmaxf(kf, 0.0f), g.C - 1.0f);
   return static_cast<int>(kf);
 }
 
@@ -393,19 +517,26 @@ __global__ __launch_bounds__(kThreads3) void k_voxels3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
     const float* __restrict__ sino, float* __restrict__ out) {
+  constexpr int S = slices3<BT>();
   const int ntx = (g.W + kTile3 - 1) / kTile3;
   const int col = (blockIdx.x % ntx) * kTile3 + (threadIdx.x % kTile3);
   const int row = (blockIdx.x / ntx) * kTile3 + (threadIdx.x / kTile3);
-  const int sl = blockIdx.y;
+  const int s0 = blockIdx.y * S;
   const int b0 = blockIdx.z * BT;
   if (col >= g.W || row >= g.H) return;
   float xcw = 0.0f, ycw = 0.0f;
   if constexpr (WM == W3_BPW) pixel_centre3(g, 1, row, col, &xcw, &ycw);
-  float acc[BT];
+  float acc[S][BT];
 #pragma unroll
-  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int b = 0; b < BT; ++b) acc[j][b] = 0.0f;
+  // world z extent of this thread's slices (block-uniform)
+  const float z0w = -g.Dz * 0.5f * g.dz + (g.scan == DD_SPIRAL ? 0.0f : g.shz);
+  const float zc_lo = z0w + s0 * g.dz, zc_hi = z0w + min(s0 + S, g.Dz) * g.dz;
   for (int v = 0; v < g.V; ++v) {
     const ViewGeo3 vg = views[v];
+    if (zc_hi < vg.zlo || zc_lo > vg.zhi) continue;  // the view's cone misses the chunk
     int l, i, N;
     if (vg.xaxis) {
       l = row;
@@ -419,38 +550,58 @@ __global__ __launch_bounds__(kThreads3) void k_voxels3d(
     const LineGeo3 lg = lines[static_cast<size_t>(v) * g.Lmax + l];
     const double* __restrict__ ce = cedge + static_cast<size_t>(v) * (g.C + 1);
     const double* __restrict__ mv = mag + static_cast<size_t>(v) * g.C;
-    int k = guess_col(g, vg, lg.a0 + (i + 0.5) * static_cast<double>(lg.p));
+    int k = guess_col_fast(g, vg, lg.a0 + (i + 0.5) * static_cast<double>(lg.p));
     while (k > 0 && edge_pos(ce[k], lg.a0, lg.inv_p, N).c >= i) --k;
     while (k < g.C && edge_pos(ce[k + 1], lg.a0, lg.inv_p, N).c < i) ++k;
     if (k >= g.C) continue;
     float gw = (WM == W3_FP) ? lg.p * lg.pzf : vg.kbp;
     if constexpr (WM == W3_BPW) gw *= fbp_weight3(g, vg, xcw, ycw);
-    const double zc = fma(static_cast<double>(sl) + 0.5, lg.pz, lg.z0);
     EdgePos p1 = edge_pos(ce[k], lg.a0, lg.inv_p, N);
     while (p1.c <= i) {
       const EdgePos p2 = edge_pos(ce[k + 1], lg.a0, lg.inv_p, N);
       const float wx = overlap_in_pixel(p1, p2, i) * gw;
-      const double m = mv[k];
-      // first detector row worth testing: invert the row-z law at the
-      // slice centre, then fix it exactly with edge_pos
-      double rf = (((zc - vg.sz) / m + vg.sz) - vg.zdet0) / vg.rowp;
-      rf = fmin(fmax(rf, 0.0), g.R - 1.0);
-      int r = static_cast<int>(rf);
-      while (r > 0 && edge_pos(row_z(vg, m, r), lg.z0, lg.inv_pz, g.Dz).c >= sl) --r;
-      while (r < g.R && edge_pos(row_z(vg, m, r + 1), lg.z0, lg.inv_pz, g.Dz).c < sl) ++r;
-      if (r < g.R) {
-        EdgePos q1 = edge_pos(row_z(vg, m, r), lg.z0, lg.inv_pz, g.Dz);
-        const float* __restrict__ sv =
-            sino + ((static_cast<size_t>(v) * g.R) * g.C + k) * g.Bp + b0;
-        while (q1.c <= sl) {
-          const EdgePos q2 = edge_pos(row_z(vg, m, r + 1), lg.z0, lg.inv_pz, g.Dz);
-          const float w = overlap_in_pixel(q1, q2, sl) * wx;
-          float q[BT];
-          load_tile<BT>(sv + static_cast<size_t>(r) * g.C * g.Bp, q);
+      const ZAffine z = z_affine(vg, lg, mv[k]);
+      // Slice sl receives sum_r Q-weighted row overlaps = B (L(u_sl+1) -
+      // L(u_sl)), where u = (t - A) / B is the detector-row coordinate of
+      // slice edge t and L(u) interpolates the row prefix Q linearly.  u is
+      // kept relative to the chunk's first row so float holds it exactly
+      // enough.
+      const double u0 = (static_cast<double>(s0) - z.A) / z.B;
+      const double rb = floor(u0);
+      const int rbase = static_cast<int>(fmin(fmax(rb, -1.0e6), 1.0e6));
+      const float du = static_cast<float>(u0 - rb);
+      const float invb = static_cast<float>(1.0 / z.B);
+      const float ulo = static_cast<float>(-rbase), uhi = static_cast<float>(g.R - rbase);
+      if (uhi > 0.0f && fmaf(static_cast<float>(S), invb, du) > ulo) {
+        const float wxb = wx * static_cast<float>(z.B);
+        const float* __restrict__ qk = sino + (static_cast<size_t>(v) * (g.R + 1) * g.C + k) * g.Bp + b0;
+        const size_t rstride = static_cast<size_t>(g.C) * g.Bp;
+        auto eval = [&](int j, float (&l)[BT]) {
+          const float u = fminf(fmaxf(fmaf(static_cast<float>(j), invb, du), ulo), uhi);
+          const float fu = floorf(u);
+          int r = rbase + static_cast<int>(fu);
+          float fr = u - fu;
+          if (r >= g.R) {  // at the top edge: the last row's end
+            r = g.R - 1;
+            fr = 1.0f;
+          }
+          float q0[BT], q1[BT];
+          load_tile<BT>(qk + r * rstride, q0);
+          load_tile<BT>(qk + (r + 1) * rstride, q1);
 #pragma unroll
-          for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
-          q1 = q2;
-          if (++r >= g.R) break;
+          for (int b = 0; b < BT; ++b) l[b] = fmaf(fr, q1[b] - q0[b], q0[b]);
+        };
+        float lprev[BT];
+        eval(0, lprev);
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          float lnext[BT];
+          eval(j + 1, lnext);
+#pragma unroll
+          for (int b = 0; b < BT; ++b) {
+            acc[j][b] = fmaf(wxb, lnext[b] - lprev[b], acc[j][b]);
+            lprev[b] = lnext[b];
+          }
         }
       }
       p1 = p2;
@@ -458,9 +609,15 @@ __global__ __launch_bounds__(kThreads3) void k_voxels3d(
     }
   }
 #pragma unroll
-  for (int b = 0; b < BT; ++b)
-    if (b0 + b < g.B)
-      out[((static_cast<size_t>(b0 + b) * g.Dz + sl) * g.H + row) * g.W + col] = acc[b];
+  for (int j = 0; j < S; ++j) {
+    const int sl = s0 + j;
+    if (sl < g.Dz) {
+#pragma unroll
+      for (int b = 0; b < BT; ++b)
+        if (b0 + b < g.B)
+          out[((static_cast<size_t>(b0 + b) * g.Dz + sl) * g.H + row) * g.W + col] = acc[j][b];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -502,7 +659,7 @@ Layout3 make_layout3(const Geo3& g, int op) {
     L.buf0 = take(sizeof(float) * vox);
     L.buf1 = take(sizeof(float) * vox);
   } else {
-    L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * g.R * g.C * g.Bp);
+    L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * (g.R + 1) * g.C * g.Bp);
     L.buf1 = L.buf0;
   }
   L.data_total = off;
@@ -556,8 +713,10 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
     hipLaunchKernelGGL(k_interleave_volume<BT>,
                        dim3((g.W + kThreads3 - 1) / kThreads3 * g.H, g.Dz, ntile),
                        dim3(kThreads3), 0, st, g, input, buf0, buf1);
-    const int nrc = (g.R + kRowsPerBlock - 1) / kRowsPerBlock;
-    dim3 grid(g.V, (g.C + kColsPerBlock - 1) / kColsPerBlock, nrc * ntile);
+    constexpr int S = rows3<BT>();
+    const int nch = (g.R + S - 1) / S;
+    const int ncb = (nch + kRowsPerBlock - 1) / kRowsPerBlock;
+    dim3 grid(g.V, (g.C + kColsPerBlock - 1) / kColsPerBlock, ncb * ntile);
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ)
       hipLaunchKernelGGL((k_bins3d<BT, W3_FP>), grid, dim3(kThreads3), 0, st, g,
@@ -569,15 +728,16 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
       hipLaunchKernelGGL((k_bins3d<BT, W3_BPW>), grid, dim3(kThreads3), 0, st, g,
                          views, cedge, mag, lines, buf0, buf1, output);
   } else {
-    dim3 sgrid((g.C + kThreads3 - 1) / kThreads3 * g.R * g.V, 1, ntile);
+    dim3 sgrid((g.C + kThreads3 - 1) / kThreads3 * g.V, 1, ntile);
     if (op == DD_PROJ_T)
-      hipLaunchKernelGGL((k_interleave_sino3d<BT, true>), sgrid, dim3(kThreads3), 0,
+      hipLaunchKernelGGL((k_prefix_sino3d<BT, true>), sgrid, dim3(kThreads3), 0,
                          st, g, views, cedge, mag, input, buf0);
     else
-      hipLaunchKernelGGL((k_interleave_sino3d<BT, false>), sgrid, dim3(kThreads3), 0,
+      hipLaunchKernelGGL((k_prefix_sino3d<BT, false>), sgrid, dim3(kThreads3), 0,
                          st, g, views, cedge, mag, input, buf0);
     const int ntx = (g.W + kTile3 - 1) / kTile3, nty = (g.H + kTile3 - 1) / kTile3;
-    dim3 grid(ntx * nty, g.Dz, ntile);
+    constexpr int S = slices3<BT>();
+    dim3 grid(ntx * nty, (g.Dz + S - 1) / S, ntile);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
     if (op == DD_PROJ_T)
       hipLaunchKernelGGL((k_voxels3d<BT, W3_FP>), grid, dim3(kThreads3), 0, st, g,

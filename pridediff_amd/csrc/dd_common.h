@@ -37,15 +37,20 @@ struct EdgePos {
   float f;
 };
 
-__device__ __forceinline__ EdgePos edge_pos(double e, double a0, double inv_p,
-                                            int n) {
-  double t = (e - a0) * inv_p;
+// Position t (in pixel units from the first pixel edge) split into pixel
+// index and fraction, clamped to [-2, n + 2].
+__device__ __forceinline__ EdgePos edge_pos_t(double t, int n) {
   t = fmin(fmax(t, -2.0), static_cast<double>(n) + 2.0);
   double fl = floor(t);
   EdgePos r;
   r.c = static_cast<int>(fl);
   r.f = static_cast<float>(t - fl);
   return r;
+}
+
+__device__ __forceinline__ EdgePos edge_pos(double e, double a0, double inv_p,
+                                            int n) {
+  return edge_pos_t((e - a0) * inv_p, n);
 }
 
 // Overlap, in pixel units, of the bin [p1, p2] with pixel c (c1 <= c <= c2).
