@@ -47,9 +47,11 @@ def parse():
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (reference kernel for HBM counters)")
-    ap.add_argument("--workload", default="fanarc", choices=["fanarc", "cone_flat", "helical"],
+    ap.add_argument("--workload", default="fanarc",
+                    choices=["fanarc", "cone_flat", "helical", "fidelity"],
                     help="fanarc: 512^2 fan-arc batch 16 (headline); cone_flat: 256^3 cone-flat "
-                         "batch 2; helical: 512x512x64 helical batch 4")
+                         "batch 2; helical: 512x512x64 helical batch 4; fidelity: PrideDiff's "
+                         "data-consistency step x - w FBP(A x - y) on the fan-arc batch")
     return ap.parse_args()
 
 
@@ -254,6 +256,61 @@ def bench3d(torch, kind, steps, warmup, check):
     return out
 
 
+def bench_fidelity(torch, steps, warmup, check):
+    """PrideDiff's data-consistency step x - w * FBP(A x - y)
+    (model/pridediff.py:56-68) on the 512^2 fan-arc batch of 16:
+    planned path in one HIP graph vs the drop-in module called on the batch
+    (GEMM filter) vs the same with the reference's conv2d filter."""
+    from pridediff_amd.autotune import tune_uniform
+    from pridediff_amd.batched import GraphStep
+    from pridediff_amd.fbp import PlannedFBP2d, fidelity_step
+
+    dev = torch.device("cuda:0")
+    P, ang, ang_np, x_true, x0, y = workload(torch, dev)
+    BP = tune_uniform(P, ang, B)
+    fbp = PlannedFBP2d(P, ang, B, plan=BP)
+    w = torch.tensor(0.5, device=dev)
+    with torch.no_grad():
+        step = GraphStep(lambda a, c: fidelity_step(BP.projection, fbp, a, c, w), x0, y)
+        t_graph = timed(torch, lambda: step.replay(x0, y), steps, warmup)
+        out_graph = step.replay(x0, y).clone()
+        t_module = timed(torch, lambda: x0 - w * P.filtered_backprojection(
+            P.projection(x0, ang) - y, ang), max(3, steps // 4), 1)
+        out_module = x0 - w * P.filtered_backprojection(P.projection(x0, ang) - y, ang)
+
+        def conv_fbp(sino):
+            f = torch.nn.functional.conv2d(sino * P.weight, P.filter, padding=(0, D - 1))
+            r = P.backprojection_w(f, ang) * (ang[-1] - ang[0]).abs() / (V - 1)
+            return r / 2
+        t_conv = timed(torch, lambda: x0 - w * conv_fbp(P.projection(x0, ang) - y), 3, 1)
+        out_conv = x0 - w * conv_fbp(P.projection(x0, ang) - y)
+        # filter stage alone: GEMM vs conv2d
+        r = (P.projection(x0, ang) - y).contiguous()
+        t_gemm_only = timed(torch, lambda: fbp.filter(r), 10, 2)
+        t_conv_only = timed(torch, lambda: torch.nn.functional.conv2d(
+            r * P.weight, P.filter, padding=(0, D - 1)), 3, 1)
+    samples = B * V * D
+    out = {
+        "metric": "PrideDiff data-consistency steps: detector samples/s",
+        "value": samples / t_graph, "unit": "detector samples/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": t_graph * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipse phantoms, Poisson noise at 1e5 photons)",
+        "config": {"workload": "x - w FBP(A x - y), fan-arc 512x512, 720 views, 736 bins, batch 16",
+                   "image": [N, N], "views": V, "bins": D, "batch": B},
+        "module_call_ms": t_module * 1e3,
+        "module_call_conv2d_filter_ms": t_conv * 1e3,
+        "filter_stage_ms": {"gemm": t_gemm_only * 1e3, "conv2d": t_conv_only * 1e3},
+        "agreement": {
+            "graph_vs_module_rel_l2": float((out_graph - out_module).double().norm()
+                                            / out_module.double().norm()),
+            "graph_vs_conv2d_rel_l2": float((out_graph - out_conv).double().norm()
+                                            / out_conv.double().norm())},
+        "cpu_baseline": None,
+    }
+    return out
+
+
 def cpu_baseline(ang_np, x_np, y_np):
     """Reference CPU code (oracle/_ref) on 2 slices: FP + ProjTrans, median of 3."""
     from oracle import oracle as O
@@ -289,6 +346,10 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.workload == "fidelity":
+        print(json.dumps(bench_fidelity(torch, args.steps, args.warmup, not args.no_accuracy)),
+              flush=True)
+        return
     if args.workload != "fanarc":
         print(json.dumps(bench3d(torch, args.workload, args.steps, args.warmup,
                                  not args.no_accuracy)), flush=True)

@@ -5,7 +5,7 @@ built once on the device, workspaces are allocated up front, and each op is
 two stream-ordered launches (layout + gather) over every instance of the
 batch at once, sharing one geometry evaluation across up to 16 instances.
 Everything it launches is capturable, so a whole data-consistency step
-(``A x``, residual, ``A^T``) can be recorded once with ``capture_step`` and
+(``A x``, residual, ``A^T``) can be recorded once with ``GraphStep`` and
 replayed as a HIP graph.
 
 No counterpart in the reference, which re-derives geometry and re-creates a

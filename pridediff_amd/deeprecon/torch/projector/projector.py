@@ -8,16 +8,20 @@ dispatch table (:8-53), FBP filter and cosine-weight generation and the FBP
 composition (sino * weight -> conv with the filter -> weighted
 backprojection -> * angular step, halved when redundant).
 
-One behavioural addition: the ops receive a host-side copy of ``options``
+Two behavioural additions: the ops receive a host-side copy of ``options``
 (refreshed whenever the parameter changes), so no op has to read geometry
-back from the device.  ``self.options`` stays a non-trainable Parameter, as
-in the reference, and moves with the module.
+back from the device, and the FBP filter stage (the reference's conv2d /
+conv3d with 2D-1 taps and D-1 padding) runs as the equal product with the
+D x D Toeplitz matrix of the taps, one library GEMM.  ``self.options`` stays
+a non-trainable Parameter, as in the reference, and moves with the module.
 """
 from typing import List, Tuple, Union
 
 import torch
 import torch.nn as nn
 from torch import Tensor
+
+from pridediff_amd.fbp import filter_sino
 
 from . import function as F
 from .filter import filter_generate
@@ -197,7 +201,7 @@ class projector2d(_ProjectorBase):
             taps = self.filter
         else:
             taps = self.filter_generate(filter_type=filter_type.lower(), dtype=sino.dtype).to(sino.device)
-        filtered = nn.functional.conv2d(sino * self.weight, taps, padding=(0, self.num_det - 1))
+        filtered = filter_sino(sino, self.weight, taps)
         recon = self.backprojection_w(filtered, angles)
         recon = recon * (angles[-1] - angles[0]).abs() / (angles.size(0) - 1)
         return recon / 2 if redundant else recon
@@ -310,7 +314,7 @@ class projector3d(_Projector3dBase):
             taps = self.filter
         else:
             taps = self.filter_generate(filter_type=filter_type.lower(), dtype=sino.dtype).to(sino.device)
-        filtered = nn.functional.conv3d(sino * self.weight, taps, padding=(0, 0, self.num_det[0] - 1))
+        filtered = filter_sino(sino, self.weight, taps)
         recon = self.backprojection_w(filtered, angles)
         recon = recon * (angles[-1] - angles[0]).abs() / (angles.size(0) - 1)
         return recon / 2 if redundant else recon
@@ -393,6 +397,6 @@ class projectorsp(_Projector3dBase):
             taps = self.filter
         else:
             taps = self.filter_generate(filter_type=filter_type.lower(), dtype=sino.dtype).to(sino.device)
-        filtered = nn.functional.conv3d(sino * self.weight, taps, padding=(0, 0, self.num_det[0] - 1))
+        filtered = filter_sino(sino, self.weight, taps)
         recon = self.backprojection_w(filtered, angles, source_pos)
         return recon * abs(angles[1] - angles[0]) / 2

@@ -1,0 +1,93 @@
+"""Filtered backprojection and PrideDiff's data-consistency step.
+
+The reference composes FBP as (deeprecon/deeprecon/torch/projector/
+projector.py:192-213, filters filter.py:12-30, weights :175-190)
+
+    recon = BackProjWeighted(conv2d(sino * weight, taps, padding=(0, D-1)))
+    recon = recon * |theta_last - theta_0| / (V - 1)   [/ 2 if redundant]
+
+and PrideDiff's fidelity module (model/pridediff.py:56-68) runs
+``x - w * FBP(A x - y)`` once per sampler step.
+
+A 2D-1 tap correlation with D-1 zeros of padding on each side maps a
+detector row s to  out[k] = sum_j s[j] taps[j - k + D - 1],  i.e. the product
+with the D x D Toeplitz matrix T[j, k] = taps[j - k + D - 1].  The cosine
+weight is a diagonal on the left and the angular step a scalar, so the whole
+filter stage is one GEMM
+
+    filtered[B*C*V, D] = sino[B*C*V, D] @ M,   M = diag(weight) T * scale,
+
+run by the FP32 library GEMM, followed by the weighted backprojection
+gather.  3D weights vary over detector rows, so there the weight is applied
+elementwise and the GEMM uses T alone.
+"""
+from __future__ import annotations
+
+import torch
+from torch import Tensor
+
+from pridediff_amd.batched import BatchedProjector2d
+
+
+def toeplitz(taps: Tensor, num_det: int) -> Tensor:
+    """D x D matrix T[j, k] = taps[j - k + D - 1] (differentiable in taps)."""
+    taps = taps.reshape(-1)
+    assert taps.numel() == 2 * num_det - 1, "filter must have 2 * num_det - 1 taps"
+    j = torch.arange(num_det, device=taps.device)
+    return taps[j[:, None] - j[None, :] + (num_det - 1)]
+
+
+def filter_matrix(weight: Tensor, taps: Tensor, num_det: int, scale=None) -> Tensor:
+    """M = diag(weight) T [* scale] for a per-detector weight of length D."""
+    m = weight.reshape(-1, 1) * toeplitz(taps, num_det)
+    return m if scale is None else m * scale
+
+
+def filter_sino(sino: Tensor, weight: Tensor, taps: Tensor) -> Tensor:
+    """``conv2d(sino * weight, taps, padding=(0, D-1))`` (2D) or the conv3d
+    counterpart (3D, weight [rows, cols]) as one GEMM over detector rows."""
+    d = sino.shape[-1]
+    if weight.dim() == 1 or weight.numel() == d:
+        m = filter_matrix(weight.reshape(-1), taps, d)
+        return (sino.reshape(-1, d) @ m).reshape(sino.shape)
+    t = toeplitz(taps, d)
+    return ((sino * weight).reshape(-1, d) @ t).reshape(sino.shape)
+
+
+def angular_scale(angles: Tensor, redundant: bool = True) -> Tensor:
+    s = (angles[-1] - angles[0]).abs() / (angles.size(0) - 1)
+    return s / 2 if redundant else s
+
+
+class PlannedFBP2d:
+    """FBP of a fixed 2D geometry, angle set and batch on the planned path:
+    one FP32 GEMM (weight, ramp filter and angular scale folded into M) and
+    one weighted-backprojection launch for the whole batch.  Differentiable
+    in the sinogram (the backward is the transposed GEMM and
+    BackProjTransWeighted).  The filter is taken once, at construction."""
+
+    def __init__(self, projector, angles: Tensor, batch: int, filter: Tensor | None = None,
+                 redundant: bool = True, tile: int = 0, variant: int = -1,
+                 plan: BatchedProjector2d | None = None):
+        taps = (projector.filter if filter is None else filter).detach()
+        self.num_det = projector.num_det
+        self.plan = plan if plan is not None else BatchedProjector2d(projector, angles, batch,
+                                                                     tile=tile, variant=variant)
+        scale = angular_scale(angles.to(torch.float32), redundant)
+        w = projector.weight.detach().to(angles.device, torch.float32)
+        self.matrix = filter_matrix(w, taps.to(angles.device, torch.float32), self.num_det,
+                                    scale).contiguous()
+
+    def filter(self, sino: Tensor) -> Tensor:
+        d = self.num_det
+        return (sino.reshape(-1, d) @ self.matrix).reshape(sino.shape)
+
+    def __call__(self, sino: Tensor) -> Tensor:
+        return self.plan.backprojection_w(self.filter(sino))
+
+
+def fidelity_step(project, fbp, x: Tensor, y: Tensor, w) -> Tensor:
+    """PrideDiff's data-consistency update  x - w * FBP(A x - y)
+    (model/pridediff.py:56-68); ``project`` and ``fbp`` are callables, e.g.
+    ``BatchedProjector2d.projection`` and a ``PlannedFBP2d``."""
+    return x - w * fbp(project(x) - y)
