@@ -88,6 +88,18 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
                    float* output, const float* angles, int batch, int views,
                    void* ws, size_t ws_bytes, void* stream);
 
+/* View-resolved sinogram -> image op (DD_PROJ_T, DD_BPROJ or DD_BPROJ_W):
+ * output [batch][views][H][W] holds each view's contribution alone, equal
+ * bit for bit to dd2d_run of that op on the one view.  Replaces the per-view
+ * loop of single-view backprojections in deeprecon's bprj_sv_fun
+ * (deeprecon/deeprecon/torch/model/vvbp_tensor_net.py:11-24), one launch for
+ * all views.  views <= 65535. */
+size_t dd2d_views_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
+                                 int views);
+dd_status dd2d_run_views(int scan, int op, const dd_geom2d* g, const float* input,
+                         float* output, const float* angles, int batch, int views,
+                         void* ws, size_t ws_bytes, void* stream);
+
 /* System matrix with the FP weights as COO entries (row = v*D + bin,
  * column = row*W + col), built on the device in two passes: count the
  * entries of every (view, bin) into `counts` [views*D], let the caller turn

@@ -87,6 +87,33 @@ __global__ __launch_bounds__(kThreads) void k_pixels(
   pixels_body<BT, WM>(g, views, edges, lines, pint, sino, out, r, c, blockIdx.z * BT);
 }
 
+// View-resolved sinogram -> image: out[b][v] is view v's contribution
+// alone (the per-view loop of deeprecon's bprj_sv_fun as one launch).
+template <int BT, int WM>
+__global__ __launch_bounds__(kThreads) void k_pixels_views(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out) {
+  const int ntx = (g.W + kTile - 1) / kTile;
+  const int c = (blockIdx.x % ntx) * kTile + (threadIdx.x % kTile);
+  const int r = (blockIdx.x / ntx) * kTile + (threadIdx.x / kTile);
+  const int v = blockIdx.y;
+  const int b0 = blockIdx.z * BT;
+  if (c >= g.W || r >= g.H) return;
+  float xcw = 0.0f, ycw = 0.0f;
+  if constexpr (WM == W_BPW) pixel_centre(g, 1, r, c, &xcw, &ycw);
+  float acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  pixel_one_view<BT, WM>(g, views[v], v, edges, lines, pint, sino, r, c, b0, xcw, ycw, acc);
+  const size_t plane = static_cast<size_t>(g.H) * g.W;
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    if (b0 + b < g.B)
+      out[(static_cast<size_t>(b0 + b) * g.V + v) * plane + static_cast<size_t>(r) * g.W + c] =
+          acc[b];
+}
+
 // Variants DD_ADJ_TILE16 / DD_ADJ_TILE8: a T x T tile stages its lines' edge
 // positions per view in LDS.
 template <int BT, int WM, int T>
@@ -331,6 +358,32 @@ dd_status run_tiles(const Geo2& q, int op, int variant, const float* input,
   }
 }
 
+template <int BT>
+dd_status launch_views(const Geo2& g, int op, const float* input, float* output,
+                       const GeoPtrs& G, char* ws, const Layout& L, hipStream_t st) {
+  auto* buf0 = reinterpret_cast<float*>(ws + L.buf0);
+  const int ntile = g.Bp / BT;
+  dim3 sgrid((g.D + kThreads - 1) / kThreads, g.V, ntile);
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_interleave_sino<BT, true>), sgrid, dim3(kThreads), 0, st, g,
+                       G.views, G.wfp, input, buf0);
+  else
+    hipLaunchKernelGGL((k_interleave_sino<BT, false>), sgrid, dim3(kThreads), 0, st, g,
+                       G.views, G.wfp, input, buf0);
+  dim3 grid(((g.W + kTile - 1) / kTile) * ((g.H + kTile - 1) / kTile), g.V, ntile);
+  ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_pixels_views<BT, W_FP>), grid, dim3(kThreads), 0, st, g, G.views,
+                       G.edges, G.lines, G.pint, buf0, output);
+  else if (op == DD_BPROJ)
+    hipLaunchKernelGGL((k_pixels_views<BT, W_BP>), grid, dim3(kThreads), 0, st, g, G.views,
+                       G.edges, G.lines, G.pint, buf0, output);
+  else
+    hipLaunchKernelGGL((k_pixels_views<BT, W_BPW>), grid, dim3(kThreads), 0, st, g, G.views,
+                       G.edges, G.lines, G.pint, buf0, output);
+  return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+}
+
 dd_status check_op(int scan, int op) {
   if (scan < DD_PARA || scan > DD_FAN_ARC) return DD_BAD_ARG;
   if (op < DD_PROJ || op > DD_BPROJ_TW) return DD_BAD_ARG;
@@ -379,6 +432,39 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   launch_setup(q, angles, w, L, st);
   return run_tiles(q, op, g_adjoint_variant, input, output, geo_ptrs(w, L), w + L.geo_total,
                    L, st);
+}
+
+dd_status dd2d_run_views(int scan, int op, const dd_geom2d* g, const float* input,
+                         float* output, const float* angles, int batch, int views,
+                         void* ws, size_t ws_bytes, void* stream) {
+  dd_status rc = check_op(scan, op);
+  if (rc != DD_OK) return rc;
+  if (op != DD_PROJ_T && op != DD_BPROJ && op != DD_BPROJ_W) return DD_UNSUPPORTED;
+  if (!input || !output || !angles) return DD_BAD_ARG;
+  if (views > 65535) return DD_UNSUPPORTED;  // one grid row per view
+  Geo2 q;
+  const int tile = std::min(pick_tile(batch), 4);
+  if (!fill_geo(&q, scan, g, batch, views, tile)) return DD_BAD_ARG;
+  const Layout L = make_layout(q, op);
+  if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
+  auto st = static_cast<hipStream_t>(stream);
+  char* w = static_cast<char*>(ws);
+  launch_setup(q, angles, w, L, st);
+  const GeoPtrs G = geo_ptrs(w, L);
+  switch (q.BT) {
+    case 1: return launch_views<1>(q, op, input, output, G, w + L.geo_total, L, st);
+    case 2: return launch_views<2>(q, op, input, output, G, w + L.geo_total, L, st);
+    default: return launch_views<4>(q, op, input, output, G, w + L.geo_total, L, st);
+  }
+}
+
+size_t dd2d_views_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
+                                 int views) {
+  Geo2 q;
+  if (check_op(scan, op) != DD_OK) return 0;
+  if (!fill_geo(&q, scan, g, batch, views, std::min(pick_tile(batch), 4))) return 0;
+  const Layout L = make_layout(q, op);
+  return L.geo_total + L.data_total;
 }
 
 size_t dd2d_geometry_size(int scan, const dd_geom2d* g, int views) {

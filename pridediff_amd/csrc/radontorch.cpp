@@ -200,6 +200,36 @@ torch::Tensor op2d(int scan, int op, const torch::Tensor& input,
   return out;
 }
 
+// View-resolved sinogram -> image op: [B, C, V, D] -> [B, C*V, H, W], entry
+// (b, c*V + v) holding view v's contribution alone.
+torch::Tensor views2d(int64_t scan, int64_t op, const torch::Tensor& input,
+                      const torch::Tensor& options, const torch::Tensor& angles) {
+  const char* name = "views2d";
+  TORCH_CHECK(scan >= DD_PARA && scan <= DD_FAN_ARC, name, ": 2D scan expected");
+  TORCH_CHECK(op == DD_PROJ_T || op == DD_BPROJ || op == DD_BPROJ_W, name,
+              ": op must be PROJ_T, BPROJ or BPROJ_W");
+  check_data(input, angles, name);
+  const dd_geom2d g = decode2d(options, name);
+  TORCH_CHECK(input.dim() == 4, name, ": expected a 4D [B, C, V, D] sinogram");
+  const int64_t B = input.size(0), C = input.size(1), V = angles.size(0);
+  TORCH_CHECK(input.size(2) == V && input.size(3) == g.num_det, name, ": sinogram is ",
+              input.sizes(), " but there are ", V, " angles and ", g.num_det,
+              " detector bins");
+  c10::hip::HIPGuard guard(input.device().index());
+  auto out = torch::empty({B, C * V, g.height, g.width}, input.options());
+  if (B * C == 0 || V == 0) return out.zero_();
+  const size_t wsb = dd2d_views_workspace_size(static_cast<int>(scan), static_cast<int>(op), &g,
+                                               static_cast<int>(B * C), static_cast<int>(V));
+  TORCH_CHECK(wsb > 0, name, ": invalid geometry or op");
+  auto ws = workspace(wsb, input);
+  DD_CHECK(dd2d_run_views(static_cast<int>(scan), static_cast<int>(op), &g,
+                          input.data_ptr<float>(), out.data_ptr<float>(),
+                          angles.data_ptr<float>(), static_cast<int>(B * C),
+                          static_cast<int>(V), ws.data_ptr(), wsb, cur_stream(input)),
+           name);
+  return out;
+}
+
 torch::Tensor sysmatrix2d(int scan, const torch::Tensor& options,
                           const torch::Tensor& angles, const char* name) {
   const dd_geom2d g = decode2d(options, name);
@@ -590,6 +620,8 @@ PYBIND11_MODULE(radontorch, m) {
     dd_profile_read(cls, &ms, &n);
     return pybind11::make_tuple(ms, n);
   });
+  m.def("views2d", &views2d, pybind11::arg("scan"), pybind11::arg("op"), pybind11::arg("input"),
+        pybind11::arg("options"), pybind11::arg("angles"));
   m.def("set_instance_tile", [](int t) { dd2d_set_instance_tile(t); });
   m.def("set_adjoint_variant", [](int v) { dd2d_set_adjoint_variant(v); });
   m.attr("ADJ_PER_PIXEL") = static_cast<int>(DD_ADJ_PER_PIXEL);
