@@ -21,6 +21,7 @@ struct ViewGeo {
   int xaxis;         // 1: common axis y = 0, lines are image rows
   int flip;          // detector index runs against the walk order
   float sxf, syf;    // source position in float, for FBP distance weights
+  float cref, sref;  // cos, sin of theta_ref in float (first-bin guesses)
 };
 
 // Per-(view, line) pixel-edge map on the common axis.
