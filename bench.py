@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 oracle check")
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven-batch measurement")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner sweep (automatic tile, default kernel variant); "
+                         "for profiling runs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--workload", default="fanarc",
@@ -83,7 +86,7 @@ def timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def uneven_c2(torch, dev, steps, warmup, check):
+def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
     """64 fan-arc instances, N in {256,384,512}, V in [96,720], D = ceil(1.44 N):
     one persistent launch per op inside one HIP graph vs 64 plain calls."""
     from pridediff_amd.autotune import tune_uneven
@@ -104,7 +107,8 @@ def uneven_c2(torch, dev, steps, warmup, check):
         imgs.append(torch.from_numpy(ellipses2d(n, 1, seed=3000 + i, scale=0.04))[0].to(dev))
     ub = UnevenBatch2d(projs, angs)
     tune = {}
-    tune_uneven(ub, report=tune)
+    if tune_grid:
+        tune_uneven(ub, report=tune)
     x = ub.pack_images(imgs)
     with torch.no_grad():
         y = (ub.projection(x) * 1.001).contiguous()
@@ -188,6 +192,39 @@ def bench3d(torch, kind, steps, warmup, check):
     step = GraphStep(lambda a, b: data_consistency_grad(fwd, a, b), x0.clone().requires_grad_(True), y)
     t = timed(torch, lambda: step.replay(x0, y), steps, warmup)
 
+    dc_loop = None
+    if kind == "helical":
+        # 50 descent steps on ||A x - y||^2, each one replay of a captured
+        # graph holding A x, the residual, A^T through autograd and the update;
+        # fixed step from an exact line search at the start point.
+        with torch.no_grad():
+            xs = torch.zeros_like(x_true)
+        xg = xs.clone().requires_grad_(True)
+        g0 = data_consistency_grad(fwd, xg, y).detach()
+        with torch.no_grad():
+            ag = fwd(g0)
+            alpha = float((g0 * g0).sum() / (2.0 * (ag * ag).sum()))
+
+        def update(a, c):
+            grad = data_consistency_grad(fwd, a, c)
+            return (a - alpha * grad).detach()
+        loop = GraphStep(update, xs.clone().requires_grad_(True), y)
+        losses = []
+        with torch.no_grad():
+            losses.append(float(((fwd(xs) - y) ** 2).sum()))
+        xk = xs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(50):
+            xk = loop.replay(xk, y)
+        torch.cuda.synchronize()
+        t_loop = time.perf_counter() - t0
+        with torch.no_grad():
+            losses.append(float(((fwd(xk) - y) ** 2).sum()))
+        dc_loop = {"steps": 50, "seconds": t_loop, "ms_per_step": t_loop / 50 * 1e3,
+                   "loss_start": losses[0], "loss_end": losses[1], "step_size": alpha,
+                   "start": "zero volume"}
+
     rt.profile_enable(True)
     rt.profile_read(0), rt.profile_read(1)
     for _ in range(2):
@@ -228,6 +265,8 @@ def bench3d(torch, kind, steps, warmup, check):
                     "k_voxels3d (A^T, sino->volume)": {"ms": bp_ms, "GB/s": b_alg / bp_ms / 1e6}},
         "cpu_baseline": None,
     }
+    if dc_loop is not None:
+        out["dc_loop"] = dc_loop
     if check:
         from oracle import oracle as O
         if kind == "cone_flat":
@@ -362,7 +401,12 @@ def main():
     dev = torch.device("cuda:0")
     P, ang, ang_np, x_true, x0, y = workload(torch, dev)
     tune = {}
-    BP = tune_uniform(P, ang, B, report=tune)
+    if args.no_tune:
+        from pridediff_amd.batched import BatchedProjector2d
+        BP = BatchedProjector2d(P, ang, B)
+        tune = {"skipped": True, "tile": BP.tile, "variant": BP.variant}
+    else:
+        BP = tune_uniform(P, ang, B, report=tune)
 
     # ---- batched path: graph-captured autograd step ----------------------
     step = GraphStep(lambda x, yy: data_consistency_grad(BP.projection, x, yy),
@@ -427,7 +471,7 @@ def main():
         acc["fp_max_abs_vs_fp64"] = O.max_abs(fp, want)
 
     c2 = None if args.no_c2 else uneven_c2(torch, dev, max(3, args.steps // 2), 2,
-                                           not args.no_accuracy)
+                                           not args.no_accuracy, not args.no_tune)
     cpu = None if args.no_cpu else cpu_baseline(ang_np, x_np, y_np)
     samples = B * V * D
     out = {
