@@ -438,11 +438,12 @@ def main():
     fp_ms, bp_ms = fp_ms / max(fp_n, 1), bp_ms / max(bp_n, 1)
     b_alg = 4.0 * V * (N * N + D) * B  # bytes per operator launch over the batch
     kernels = {
-        "k_bins (A, image->sino)": {"ms": fp_ms, "GB/s": b_alg / (fp_ms * 1e-3) / 1e9},
+        "k_bins_split + k_bins_reduce (A, image->sino)":
+            {"ms": fp_ms, "GB/s": b_alg / (fp_ms * 1e-3) / 1e9},
         "k_pixels (A^T, sino->image)": {"ms": bp_ms, "GB/s": b_alg / (bp_ms * 1e-3) / 1e9},
     }
     dom_ms = max(fp_ms, bp_ms)
-    dom_name = "k_bins" if fp_ms >= bp_ms else "k_pixels"
+    dom_name = "k_bins_split" if fp_ms >= bp_ms else "k_pixels"
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "fanarc_c1_hbm_traffic.json")
     if os.path.exists(pmc):

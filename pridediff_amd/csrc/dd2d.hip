@@ -4,14 +4,15 @@
 // FanFlatDisCPU2d.cpp (fan flat), ParaDisCPU2d.cpp (parallel), and the op
 // table of deeprecon/interface/RadonTorch.cpp:17-568.
 //
-// One call = four stream-ordered launches, no host synchronisation:
+// One call = four or five stream-ordered launches, no host synchronisation:
 //   1. k_setup2d        per view: axis/flip choice, bin edges on the common
 //                       axis (FP64), FP weights, per-line pixel-edge maps.
 //   2. k_interleave_*   copies the input into an instance-interleaved layout
 //                       ([line][pixel][b] or [view][bin][b]) so one geometry
 //                       evaluation feeds BT instances through vector loads.
-//   3. k_bins           image -> sinogram ops: one thread per detector bin
-//                       gathers along every image line (no atomics).
+//   3. k_bins_split     image -> sinogram ops: one thread per detector bin and
+//                       chunk of image lines gathers along its lines (no
+//                       atomics); k_bins_reduce adds the chunks in order.
 //      k_pixels         sinogram -> image ops: one thread per pixel gathers
 //                       over every view (no atomics).
 #include <hip/hip_runtime.h>
@@ -61,17 +62,64 @@ __global__ __launch_bounds__(kThreads) void k_interleave_sino(
   interleave_sino_bin<BT, FOLD_W>(g, views, wfp, p, s, blockIdx.y, k, blockIdx.z * BT);
 }
 
-// image -> sinogram: one thread per (view, bin), BT instances
+// image -> sinogram: one thread per (view, bin, chunk of lines), BT
+// instances.  Work item i has chunk i % kSplit: workgroups are dealt to the 8
+// XCDs in turn, so each XCD's L2 only holds its chunk's lines of the image.
+// Writes unweighted partial sums part[chunk][b][v][k] (walk order);
+// k_bins_reduce adds them in chunk order.
+
 template <int BT, int WM>
-__global__ __launch_bounds__(kThreads) void k_bins(
+__global__ __launch_bounds__(kThreads) void k_bins_split(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
-    const float* __restrict__ wfp, const LineGeo* __restrict__ lines,
-    const float* __restrict__ pint, const float* __restrict__ xrow,
-    const float* __restrict__ xcol, float* __restrict__ out) {
-  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ xrow, const float* __restrict__ xcol,
+    float* __restrict__ part) {
+  const int kb_n = (g.D + kThreads - 1) / kThreads;
+  const int chunk = blockIdx.x % kSplit;
+  int rest = blockIdx.x / kSplit;
+  const int kb = rest % kb_n;
+  rest /= kb_n;
+  const int v = rest % g.V;
+  const int b0 = (rest / g.V) * BT;
+  const int k = kb * kThreads + threadIdx.x;
   if (k >= g.D) return;
-  bins_body<BT, WM>(g, views, edges, wfp, lines, pint, xrow, xcol, out,
-                    blockIdx.x, k, blockIdx.z * BT);
+  const ViewGeo vg = views[v];
+  const int L = vg.xaxis ? g.H : g.W;
+  const int N = vg.xaxis ? g.W : g.H;
+  const int l0 = L * chunk / kSplit, l1 = L * (chunk + 1) / kSplit;
+  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + k];
+  const double e2 = edges[static_cast<size_t>(v) * (g.D + 1) + k + 1];
+  float acc[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+  bins_accumulate<BT, WM>(g, vg, e1, e2, lines + static_cast<size_t>(v) * g.Lmax,
+                          pint + static_cast<size_t>(v) * g.Lmax,
+                          (vg.xaxis ? xrow : xcol) + b0, bp_const(g), l0, l1, N, acc);
+#pragma unroll
+  for (int b = 0; b < BT; ++b)
+    if (b0 + b < g.B)
+      part[((static_cast<size_t>(chunk) * g.Bp + b0 + b) * g.V + v) * g.D + k] = acc[b];
+}
+
+// part [kSplit][Bp][V][D] -> out [B][V][D]: chunk sums in order, times the
+// bin's FP weight, detector order.
+template <int WM>
+__global__ __launch_bounds__(kThreads) void k_bins_reduce(
+    Geo2 g, const ViewGeo* __restrict__ views, const float* __restrict__ wfp,
+    const float* __restrict__ part, float* __restrict__ out) {
+  const int kb_n = (g.D + kThreads - 1) / kThreads;
+  const int k = (blockIdx.x % kb_n) * kThreads + threadIdx.x;
+  const int v = (blockIdx.x / kb_n) % g.V, b = blockIdx.x / (kb_n * g.V);
+  if (k >= g.D) return;
+  const size_t plane = static_cast<size_t>(g.Bp) * g.V * g.D;
+  const size_t at = (static_cast<size_t>(b) * g.V + v) * g.D + k;
+  float sum = part[at];
+#pragma unroll
+  for (int c = 1; c < kSplit; ++c) sum += part[c * plane + at];
+  const ViewGeo vg = views[v];
+  const int kk = vg.flip ? g.D - 1 - k : k;
+  const float wk = (WM == W_FP) ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
+  out[(static_cast<size_t>(b) * g.V + v) * g.D + kk] = sum * wk;
 }
 
 // sinogram -> image: one thread per pixel, BT instances, all views.
@@ -181,8 +229,10 @@ __global__ __launch_bounds__(kThreads) void k_sysmatrix(
 // ---------------------------------------------------------------------------
 struct Layout {
   size_t views, edges, wfp, lines, pint, geo_total;  // geometry tables
-  size_t buf0, buf1, data_total;                      // data buffers
+  size_t buf0, buf1, part, data_total;                // data buffers
 };
+
+
 
 constexpr size_t kAlign = 256;
 
@@ -215,6 +265,7 @@ Layout make_layout(const Geo2& g, int op) {
   if (to_sino(op)) {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
     L.buf1 = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
+    L.part = take(sizeof(float) * static_cast<size_t>(kSplit) * g.Bp * g.V * g.D);
   } else {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * g.D * g.Bp);
     L.buf1 = L.buf0;
@@ -288,17 +339,27 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
     hipLaunchKernelGGL(k_interleave_image<BT>,
                        dim3((g.W + kThreads - 1) / kThreads, g.H, ntile),
                        dim3(kThreads), 0, st, g, input, buf0, buf1);
-    dim3 grid(g.V, (g.D + kThreads - 1) / kThreads, ntile);
+    auto* part = reinterpret_cast<float*>(ws + L.part);
+    const int kb_n = (g.D + kThreads - 1) / kThreads;
+    dim3 pgrid(kSplit * kb_n * g.V * ntile);
+    dim3 rgrid(kb_n * g.V * g.B);
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
-    if (op == DD_PROJ)
-      hipLaunchKernelGGL((k_bins<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                         G.views, G.edges, G.wfp, G.lines, G.pint, buf0, buf1, output);
-    else if (op == DD_BPROJ_T)
-      hipLaunchKernelGGL((k_bins<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                         G.views, G.edges, G.wfp, G.lines, G.pint, buf0, buf1, output);
-    else
-      hipLaunchKernelGGL((k_bins<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                         G.views, G.edges, G.wfp, G.lines, G.pint, buf0, buf1, output);
+    if (op == DD_PROJ) {
+      hipLaunchKernelGGL((k_bins_split<BT, W_FP>), pgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.edges, G.lines, G.pint, buf0, buf1, part);
+      hipLaunchKernelGGL((k_bins_reduce<W_FP>), rgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.wfp, part, output);
+    } else if (op == DD_BPROJ_T) {
+      hipLaunchKernelGGL((k_bins_split<BT, W_BP>), pgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.edges, G.lines, G.pint, buf0, buf1, part);
+      hipLaunchKernelGGL((k_bins_reduce<W_BP>), rgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.wfp, part, output);
+    } else {
+      hipLaunchKernelGGL((k_bins_split<BT, W_BPW>), pgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.edges, G.lines, G.pint, buf0, buf1, part);
+      hipLaunchKernelGGL((k_bins_reduce<W_BPW>), rgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.wfp, part, output);
+    }
   } else {
     dim3 sgrid((g.D + kThreads - 1) / kThreads, g.V, ntile);
     if (op == DD_PROJ_T)

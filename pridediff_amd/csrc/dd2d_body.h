@@ -209,27 +209,18 @@ __device__ __forceinline__ void pixel_centre(const Geo2& g, int xaxis, int l,
 // ---------------------------------------------------------------------------
 // 3a. image -> sinogram: one thread per (view, bin), BT instances
 // ---------------------------------------------------------------------------
-template <int BT, int WM>
-__device__ __forceinline__ void bins_body(
-    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
-    const float* __restrict__ wfp, const LineGeo* __restrict__ lines,
-    const float* __restrict__ pint, const float* __restrict__ xrow,
-    const float* __restrict__ xcol, float* __restrict__ out, int v, int k, int b0) {
-  const ViewGeo vg = views[v];
-  const int xaxis = vg.xaxis;
-  const int L = xaxis ? g.H : g.W;
-  const int N = xaxis ? g.W : g.H;
-  const float* __restrict__ src = (xaxis ? xrow : xcol) + b0;
-  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + k];
-  const double e2 = edges[static_cast<size_t>(v) * (g.D + 1) + k + 1];
-  const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
-  const float* __restrict__ pv = pint + static_cast<size_t>(v) * g.Lmax;
-  const float kbp = bp_const(g);
-  float acc[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
+// Image -> sinogram gathers split each bin's line loop into kSplit chunks
+// (one per XCD) and add the chunk sums in chunk order in a second pass.
+constexpr int kSplit = 8;
 
-  for (int l = 0; l < L; ++l) {
+// Sum over lines [l0, l1) of one bin's overlaps with the image (no bin weight).
+template <int BT, int WM>
+__device__ __forceinline__ void bins_accumulate(
+    const Geo2& g, const ViewGeo& vg, double e1, double e2, const LineGeo* __restrict__ lv,
+    const float* __restrict__ pv, const float* __restrict__ src, float kbp, int l0, int l1,
+    int N, float (&acc)[BT]) {
+  const int xaxis = vg.xaxis;
+  for (int l = l0; l < l1; ++l) {
     const LineGeo lg = lv[l];
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
@@ -251,12 +242,6 @@ __device__ __forceinline__ void bins_body(
       for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
     }
   }
-  const int kk = vg.flip ? g.D - 1 - k : k;
-  const float wk = (WM == W_FP) ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
-#pragma unroll
-  for (int b = 0; b < BT; ++b)
-    if (b0 + b < g.B)
-      out[(static_cast<size_t>(b0 + b) * g.V + v) * g.D + kk] = acc[b] * wk;
 }
 
 // ---------------------------------------------------------------------------

@@ -5,10 +5,11 @@
 // detector, view count and angles) and its own slot in one packed image
 // buffer and one packed sinogram buffer.  setup() builds every instance's
 // FP64 geometry tables once and uploads a device descriptor table; run() is
-// then two launches per op for the whole batch — a layout phase and a gather
-// phase — each a persistent grid that walks a static list of work items
-// (instance, view, bin chunk) or (instance, pixel tile), heaviest instances
-// first.  No counters, no atomics, no host work: a run is capturable and
+// then two or three launches per op for the whole batch — a layout phase, a
+// gather phase and, for image -> sinogram ops, the chunk reduction — each a
+// persistent grid that walks a static list of work items (instance, view,
+// bin block, line chunk), (instance, view, bin block) or (instance, pixel
+// tile), heaviest instances first.  No counters, no atomics, no host work: a run is capturable and
 // replays bit-identically.
 //
 // The per-item arithmetic is the one of the one-call kernels
@@ -44,13 +45,21 @@ struct BatchInst {
   float* xrow;         // layout scratch: image rows, interleaved
   float* xcol;         // layout scratch: image columns, bottom-up
   float* sino;         // layout scratch: sinogram in walk order
+  float* part;         // partial bin sums per line chunk [kSplit][B][V][D]
   long long img_off;   // floats, into the packed image buffer
   long long sino_off;  // floats, into the packed sinogram buffer
-  int item0[4];        // first work item of each phase (see Phase)
-  int nitems[4];
+  int item0[5];        // first work item of each phase (see Phase)
+  int nitems[5];
 };
 
-enum Phase { kLayoutImage = 0, kGatherBins = 1, kLayoutSino = 2, kGatherPixels = 3 };
+enum Phase {
+  kLayoutImage = 0,
+  kGatherBins = 1,
+  kLayoutSino = 2,
+  kGatherPixels = 3,
+  kReduceBins = 4,
+  kPhases = 5
+};
 
 constexpr int kBT = 1;  // one geometry per instance
 
@@ -105,21 +114,65 @@ __global__ __launch_bounds__(kThreads) void k_batch_layout(
   }
 }
 
+// Gather-bins items: (view, bin block, instance, line chunk), chunk fastest;
+// every instance's item count and first item are multiples of kSplit, so with
+// a grid that is a multiple of 8 the chunk follows the XCD.
 template <int WM>
 __global__ __launch_bounds__(kThreads) void k_batch_bins(
-    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out) {
+    const BatchInst* __restrict__ inst, int n_inst, int total) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int i = find_inst(inst, n_inst, kGatherBins, item);
     const BatchInst& d = inst[i];
     const int local = item - d.item0[kGatherBins];
-    const int chunks = (d.g.D + kThreads - 1) / kThreads;
-    const int v = local % d.g.V;
-    const int rest = local / d.g.V;
-    const int k = (rest % chunks) * kThreads + threadIdx.x;
-    const int b0 = (rest / chunks) * kBT;
-    if (k < d.g.D)
-      bins_body<kBT, WM>(d.g, d.views, d.edges, d.wfp, d.lines, d.pint, d.xrow, d.xcol,
-                         out + d.sino_off, v, k, b0);
+    const int chunk = local % kSplit;
+    int rest = local / kSplit;
+    const int kb_n = (d.g.D + kThreads - 1) / kThreads;
+    const int k = (rest % kb_n) * kThreads + threadIdx.x;
+    rest /= kb_n;
+    const int v = rest % d.g.V;
+    const int b0 = (rest / d.g.V) * kBT;
+    if (k >= d.g.D) continue;
+    const ViewGeo vg = d.views[v];
+    const int L = vg.xaxis ? d.g.H : d.g.W;
+    const int N = vg.xaxis ? d.g.W : d.g.H;
+    const double e1 = d.edges[static_cast<size_t>(v) * (d.g.D + 1) + k];
+    const double e2 = d.edges[static_cast<size_t>(v) * (d.g.D + 1) + k + 1];
+    float acc[kBT];
+#pragma unroll
+    for (int b = 0; b < kBT; ++b) acc[b] = 0.0f;
+    bins_accumulate<kBT, WM>(d.g, vg, e1, e2, d.lines + static_cast<size_t>(v) * d.g.Lmax,
+                             d.pint + static_cast<size_t>(v) * d.g.Lmax,
+                             (vg.xaxis ? d.xrow : d.xcol) + b0, bp_const(d.g),
+                             L * chunk / kSplit, L * (chunk + 1) / kSplit, N, acc);
+#pragma unroll
+    for (int b = 0; b < kBT; ++b)
+      d.part[((static_cast<size_t>(chunk) * d.g.Bp + b0 + b) * d.g.V + v) * d.g.D + k] = acc[b];
+  }
+}
+
+// Reduce items: (view, bin block, instance): chunk sums in chunk order, times
+// the bin weight, detector order (the arithmetic of k_bins_reduce).
+template <int WM>
+__global__ __launch_bounds__(kThreads) void k_batch_reduce(
+    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(inst, n_inst, kReduceBins, item);
+    const BatchInst& d = inst[i];
+    const int local = item - d.item0[kReduceBins];
+    const int kb_n = (d.g.D + kThreads - 1) / kThreads;
+    const int k = (local % kb_n) * kThreads + threadIdx.x;
+    const int v = (local / kb_n) % d.g.V;
+    const int b = local / (kb_n * d.g.V);
+    if (k >= d.g.D) continue;
+    const size_t plane = static_cast<size_t>(d.g.Bp) * d.g.V * d.g.D;
+    const size_t at = (static_cast<size_t>(b) * d.g.V + v) * d.g.D + k;
+    float sum = d.part[at];
+#pragma unroll
+    for (int c = 1; c < kSplit; ++c) sum += d.part[c * plane + at];
+    const ViewGeo vg = d.views[v];
+    const int kk = vg.flip ? d.g.D - 1 - k : k;
+    const float wk = (WM == W_FP) ? d.wfp[static_cast<size_t>(v) * d.g.D + k] : 1.0f;
+    out[d.sino_off + (static_cast<size_t>(b) * d.g.V + v) * d.g.D + kk] = sum * wk;
   }
 }
 
@@ -159,7 +212,7 @@ struct dd2d_batch {
   std::vector<int> order;        // instance order (heaviest first)
   size_t desc_off = 0, angles_off = 0, bytes = 0;
   size_t total_views = 0;
-  int total[4] = {0, 0, 0, 0};
+  int total[kPhases] = {0, 0, 0, 0, 0};
   int grid = 0;                  // persistent workgroups (0 = automatic)
   int variant = DD_ADJ_PER_PIXEL; // sinogram -> image kernel (per pixel or 16x16 tiles)
   int cus = 0;
@@ -210,7 +263,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
   b->desc_off = take(&off, sizeof(BatchInst) * n);
   b->angles_off = take(&off, sizeof(float*) * n);
   b->inst.resize(n);
-  int acc[4] = {0, 0, 0, 0};
+  int acc[kPhases] = {0, 0, 0, 0, 0};
   for (int j = 0; j < n; ++j) {
     const auto& it = items[b->order[j]];
     BatchInst d{};
@@ -242,21 +295,24 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     d.xrow = reinterpret_cast<float*>(take(&off, sizeof(float) * img));
     d.xcol = reinterpret_cast<float*>(take(&off, sizeof(float) * img));
     d.sino = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.D * g.Bp));
+    d.part = reinterpret_cast<float*>(
+        take(&off, sizeof(float) * static_cast<size_t>(kSplit) * g.Bp * g.V * g.D));
     d.img_off = it.image_offset;
     d.sino_off = it.sino_offset;
     const int tiles = it.batch;  // kBT == 1
     d.nitems[kLayoutImage] = g.H * tiles;
-    d.nitems[kGatherBins] = g.V * ((g.D + kThreads - 1) / kThreads) * tiles;
+    d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kThreads - 1) / kThreads) * tiles;
+    d.nitems[kReduceBins] = g.V * ((g.D + kThreads - 1) / kThreads) * it.batch;
     d.nitems[kLayoutSino] = g.V * tiles;
     d.nitems[kGatherPixels] = ((g.W + kTile - 1) / kTile) * ((g.H + kTile - 1) / kTile) * tiles;
-    for (int ph = 0; ph < 4; ++ph) {
+    for (int ph = 0; ph < kPhases; ++ph) {
       d.item0[ph] = acc[ph];
       acc[ph] += d.nitems[ph];
     }
     b->total_views += g.V;
     b->inst[j] = d;
   }
-  for (int ph = 0; ph < 4; ++ph) b->total[ph] = acc[ph];
+  for (int ph = 0; ph < kPhases; ++ph) b->total[ph] = acc[ph];
   b->bytes = off;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -295,6 +351,7 @@ dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* 
     d.xrow = rebase(d.xrow);
     d.xcol = rebase(d.xcol);
     d.sino = rebase(d.sino);
+    d.part = rebase(d.part);
     angles[j] = b->items[b->order[j]].angles;
   }
   b->inst = host;
@@ -326,17 +383,24 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
     hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutImage])),
                        dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutImage),
                        b->total[kLayoutImage], input);
-    const int tot = b->total[kGatherBins];
+    const int tot = b->total[kGatherBins], rtot = b->total[kReduceBins];
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
-    if (op == DD_PROJ)
+    if (op == DD_PROJ) {
       hipLaunchKernelGGL((k_batch_bins<W_FP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
-                         inst, n, tot, output);
-    else if (op == DD_BPROJ_T)
+                         inst, n, tot);
+      hipLaunchKernelGGL((k_batch_reduce<W_FP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
+                         st, inst, n, rtot, output);
+    } else if (op == DD_BPROJ_T) {
       hipLaunchKernelGGL((k_batch_bins<W_BP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
-                         inst, n, tot, output);
-    else
+                         inst, n, tot);
+      hipLaunchKernelGGL((k_batch_reduce<W_BP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
+                         st, inst, n, rtot, output);
+    } else {
       hipLaunchKernelGGL((k_batch_bins<W_BPW>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
-                         inst, n, tot, output);
+                         inst, n, tot);
+      hipLaunchKernelGGL((k_batch_reduce<W_BPW>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
+                         st, inst, n, rtot, output);
+    }
   } else {
     if (op == DD_PROJ_T)
       hipLaunchKernelGGL((k_batch_layout<true>), dim3(grid_for(b, b->total[kLayoutSino])),
