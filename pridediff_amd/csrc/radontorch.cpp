@@ -12,9 +12,9 @@
 //    part of this build and CPU tensors raise an error (never a CPU fallback);
 //  * `options` may live on the CPU (the Python layer passes a host mirror and
 //    the op reads it without a device sync).  A GPU-resident options tensor is
-//    read once per tensor version outside capture and remembered, so a graph
-//    capture replays without host reads (the reference issues 11-17
-//    `.item()` syncs per call, RadonTorch.cpp:227-237);
+//    read once per tensor version (outside capture) and remembered, so later
+//    calls and graph captures run without host reads (the reference issues
+//    11-17 `.item()` syncs per call, RadonTorch.cpp:227-237);
 //  * input shapes are checked against the geometry instead of being read out
 //    of bounds, and every HIP error is reported.
 #include <torch/extension.h>
@@ -53,13 +53,20 @@ const char* status_name(int rc) {
   } while (0)
 
 // ---- options decoding ------------------------------------------------------
+// Device-resident options are read to the host once per (tensor, version,
+// storage) and remembered.  Each entry holds a reference to its tensor, so
+// an entry's key (the TensorImpl address) cannot be reused by another tensor
+// while it is cached; the cache is cleared when it reaches kOptCacheMax.
 struct CachedOptions {
+  torch::Tensor ref;
   int64_t version;
   const void* data;
   std::vector<float> values;
 };
+constexpr size_t kOptCacheMax = 256;
 std::mutex g_opt_mu;
-std::unordered_map<const c10::TensorImpl*, CachedOptions> g_opt_cache;
+// never destroyed: its tensors must not be freed after the HIP runtime at exit
+auto& g_opt_cache = *new std::unordered_map<const c10::TensorImpl*, CachedOptions>();
 
 bool capturing() {
   return c10::hip::currentStreamCaptureStatusMayInitCtx() !=
@@ -80,19 +87,19 @@ std::vector<float> read_options(const torch::Tensor& options, int64_t n,
   const void* data = options.data_ptr();
   std::lock_guard<std::mutex> lock(g_opt_mu);
   auto it = g_opt_cache.find(key);
-  if (capturing()) {
-    TORCH_CHECK(it != g_opt_cache.end() && it->second.version == version &&
-                    it->second.data == data,
-                op, ": a device-resident options tensor must be used once "
-                "outside stream capture before it is captured (or pass the "
-                "options on the CPU)");
+  if (it != g_opt_cache.end() && it->second.version == version && it->second.data == data &&
+      static_cast<int64_t>(it->second.values.size()) == n)
     return it->second.values;
-  }
+  TORCH_CHECK(!capturing(), op,
+              ": a device-resident options tensor must be used once outside stream "
+              "capture before it is captured (or pass the options on the CPU)");
   auto o = options.to(torch::kCPU, torch::kFloat32).contiguous();
   const float* p = o.data_ptr<float>();
-  CachedOptions c{version, data, std::vector<float>(p, p + n)};
-  g_opt_cache[key] = c;
-  return c.values;
+  if (g_opt_cache.size() >= kOptCacheMax) g_opt_cache.clear();
+  CachedOptions c{options, version, data, std::vector<float>(p, p + n)};
+  auto& slot = g_opt_cache[key];
+  slot = std::move(c);
+  return slot.values;
 }
 
 dd_geom2d decode2d(const torch::Tensor& options, const char* op) {

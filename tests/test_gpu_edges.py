@@ -153,3 +153,33 @@ def test_views2d_rejects_too_many_views(rt):
     with pytest.raises(RuntimeError):
         rt.views2d(rt.SCAN["arc"], rt.BPROJ, torch.zeros(1, 1, 65536, 9).cuda(), opt,
                    torch.zeros(65536).cuda())
+
+
+def test_device_resident_options_are_cached_and_refreshed(rt):
+    """A GPU options tensor is read once per version: reuse gives the same
+    result, an in-place change is picked up, a graph capture needs no read."""
+    g = _geom("arc", 32, 32, 45)
+    ang = torch.from_numpy((np.arange(20) * (2 * np.pi / 20)).astype(np.float32)).cuda()
+    x = torch.rand(2, 1, 32, 32).cuda()
+    opt = torch.tensor(g.options(), dtype=torch.float32).cuda()
+    host = torch.tensor(g.options(), dtype=torch.float32)
+    a = rt.ProjFanArcDis2d(x, opt, ang)
+    assert torch.equal(a, rt.ProjFanArcDis2d(x, opt, ang))
+    assert torch.equal(a, rt.ProjFanArcDis2d(x, host, ang))
+    opt[3] = opt[3] * 1.5  # pixel size: in-place change bumps the version
+    host2 = host.clone()
+    host2[3] = host2[3] * 1.5
+    b = rt.ProjFanArcDis2d(x, opt, ang)
+    assert torch.equal(b, rt.ProjFanArcDis2d(x, host2, ang))
+    assert not torch.equal(a, b)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        rt.ProjFanArcDis2d(x, opt, ang)
+    torch.cuda.current_stream().wait_stream(side)
+    with torch.cuda.graph(graph):
+        out = rt.ProjFanArcDis2d(x, opt, ang)
+    graph.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(out, b)
