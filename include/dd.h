@@ -126,11 +126,13 @@ size_t dd2d_geometry_size(int scan, const dd_geom2d* g, int views);
 dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
                         int views, void* geo, size_t geo_bytes, void* stream);
 /* Kernel variants of the sinogram -> image gathers (all exact and equal):
- * each pixel finds its bins on its own, or a 16x16 / 8x8 pixel tile stages
- * its lines' bin-edge positions per view in LDS. */
+ * each pixel finds its bins on its own; a 16x16 / 8x8 pixel tile stages its
+ * lines' bin-edge positions per view in LDS; or one thread walks the bins of
+ * a 2x2 pixel block once for its two lines of two pixels. */
 #define DD_ADJ_PER_PIXEL 0
 #define DD_ADJ_TILE16 1
 #define DD_ADJ_TILE8 2
+#define DD_ADJ_QUAD 3
 
 /* `tile` = instances sharing one geometry evaluation (1, 2, 4, 8 or 16;
  * 0 = automatic); it must be the same in both calls.  `variant` selects the
@@ -171,7 +173,8 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n);
 size_t dd2d_batch_device_bytes(const dd2d_batch* b);
 /* Persistent grid size in workgroups (0 = 8 per compute unit). */
 void dd2d_batch_set_grid(dd2d_batch* b, int workgroups);
-/* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL or DD_ADJ_TILE16. */
+/* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL, DD_ADJ_TILE16 or
+ * DD_ADJ_QUAD. */
 void dd2d_batch_set_variant(dd2d_batch* b, int variant);
 dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes,
                            void* stream);
@@ -191,7 +194,10 @@ void dd2d_batch_destroy(dd2d_batch* b);
 void dd_profile_enable(int on);
 int dd_profile_read(int kernel_class, double* total_ms, long long* launches);
 
-/* Default sinogram -> image kernel variant of dd2d_run (DD_ADJ_*). */
+/* Default sinogram -> image kernel variant of dd2d_run and of planned runs
+ * given variant -1 (DD_ADJ_*; -1 = automatic: DD_ADJ_QUAD for up to 4
+ * instances per launch when that still gives 4 workgroups per compute unit,
+ * DD_ADJ_PER_PIXEL otherwise). */
 void dd2d_set_adjoint_variant(int variant);
 
 /* Instance tile used per launch (1, 2, 4, 8 or 16 instances share one

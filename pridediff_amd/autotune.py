@@ -3,7 +3,8 @@
 Sweeps the launch parameters of a planned operator on the problem it will
 run — the instance tile (how many instances share one geometry evaluation,
 1-16) and the sinogram -> image kernel variant (per pixel, 16x16 or 8x8
-LDS-staged pixel tiles) of a uniform batch, the persistent grid size and
+tiles with LDS edge positions, or one thread per 2x2 pixels) of a uniform
+batch, the persistent grid size and
 kernel variant of an uneven batch —
 times one forward projection plus one adjoint per candidate with HIP events,
 keeps the fastest, and remembers it in a JSON cache keyed by the device and
@@ -26,7 +27,7 @@ CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
 
 
-KEY_VERSION = 2  # bumped whenever the kernels or the sweep change
+KEY_VERSION = 3  # bumped whenever the kernels or the sweep change
 
 
 def _device_key() -> str:
@@ -75,7 +76,7 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
     """Fastest ``BatchedProjector2d`` (tile, kernel variant) for this
     projector, angle set and batch."""
     if variants is None:
-        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8)
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD)
     opts = [round(float(v), 6) for v in projector.host_options().tolist()]
     key = f"{_device_key()}|uniform|{projector.scan_type}|{opts}|V={angles.numel()}|B={batch}"
     table = _load() if use_cache else {}
@@ -114,7 +115,7 @@ def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(1, 2, 4, 8, 16), variants=No
     """Sets the fastest persistent grid size and kernel variant on ``batch``
     (in place)."""
     if variants is None:
-        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16)
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD)
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
     sig = sorted(batch.shapes)
     key = f"{_device_key()}|uneven|{batch.scan_type}|{sig}"

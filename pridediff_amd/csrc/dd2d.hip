@@ -32,7 +32,31 @@ namespace {
 using namespace body2d;
 
 int g_forced_tile = 0;
-int g_adjoint_variant = DD_ADJ_PER_PIXEL;
+// sinogram -> image kernel of dd2d_run and of plans without a variant:
+// -1 = automatic: 2 x 2 quads for up to 4 instances per launch when they
+// still give at least 4 workgroups per compute unit, per pixel otherwise.
+int g_adjoint_variant = -1;
+
+int compute_units() {
+  static int cus = 0;  // first call wins; every device of a process is an MI355X here
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
+int adjoint_variant(int requested, const Geo2& g) {
+  if (requested < DD_ADJ_PER_PIXEL || requested > DD_ADJ_QUAD) requested = g_adjoint_variant;
+  if (requested >= DD_ADJ_PER_PIXEL) return requested;
+  const long long quad_groups = 1LL * ((g.W + 31) / 32) * ((g.H + 31) / 32) * (g.Bp / g.BT);
+  return (g.BT <= 4 && quad_groups >= 4LL * compute_units()) ? DD_ADJ_QUAD : DD_ADJ_PER_PIXEL;
+}
 
 // ---------------------------------------------------------------------------
 // one-call kernels: thin grids over the shared bodies
@@ -160,6 +184,19 @@ __global__ __launch_bounds__(kThreads) void k_pixels_views(
     if (b0 + b < g.B)
       out[(static_cast<size_t>(b0 + b) * g.V + v) * plane + static_cast<size_t>(r) * g.W + c] =
           acc[b];
+}
+
+// Variant DD_ADJ_QUAD: one thread per 2 x 2 pixel block, a 32 x 32 pixel
+// tile per workgroup.
+template <int BT, int WM>
+__global__ __launch_bounds__(kThreads) void k_pixels_quad(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out) {
+  const int c0 = blockIdx.x * (2 * kTile) + 2 * (threadIdx.x % kTile);
+  const int r0 = blockIdx.y * (2 * kTile) + 2 * (threadIdx.x / kTile);
+  if (c0 >= g.W || r0 >= g.H) return;
+  quads_body<BT, WM>(g, views, edges, lines, pint, sino, out, r0, c0, blockIdx.z * BT);
 }
 
 // Variants DD_ADJ_TILE16 / DD_ADJ_TILE8: a T x T tile stages its lines' edge
@@ -369,7 +406,18 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       hipLaunchKernelGGL((k_interleave_sino<BT, false>), sgrid, dim3(kThreads),
                          0, st, g, G.views, G.wfp, input, buf0);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (variant == DD_ADJ_TILE16) {
+    if (variant == DD_ADJ_QUAD) {
+      dim3 grid((g.W + 2 * kTile - 1) / (2 * kTile), (g.H + 2 * kTile - 1) / (2 * kTile), ntile);
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL((k_pixels_quad<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else if (op == DD_BPROJ)
+        hipLaunchKernelGGL((k_pixels_quad<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      else
+        hipLaunchKernelGGL((k_pixels_quad<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
+                           G.views, G.edges, G.lines, G.pint, buf0, output);
+    } else if (variant == DD_ADJ_TILE16) {
       dim3 grid((g.W + 15) / 16, (g.H + 15) / 16, ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 16>), grid, dim3(256), 0, st, g,
@@ -462,8 +510,7 @@ extern "C" {
 const char* dd_version(void) { return "libdd 0.1 (gfx950)"; }
 
 void dd2d_set_adjoint_variant(int variant) {
-  g_adjoint_variant = (variant >= DD_ADJ_PER_PIXEL && variant <= DD_ADJ_TILE8) ? variant
-                                                                               : DD_ADJ_PER_PIXEL;
+  g_adjoint_variant = (variant >= DD_ADJ_PER_PIXEL && variant <= DD_ADJ_QUAD) ? variant : -1;
 }
 
 void dd2d_set_instance_tile(int tile) {
@@ -491,8 +538,8 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
   launch_setup(q, angles, w, L, st);
-  return run_tiles(q, op, g_adjoint_variant, input, output, geo_ptrs(w, L), w + L.geo_total,
-                   L, st);
+  return run_tiles(q, op, adjoint_variant(-1, q), input, output, geo_ptrs(w, L),
+                   w + L.geo_total, L, st);
 }
 
 dd_status dd2d_run_views(int scan, int op, const dd_geom2d* g, const float* input,
@@ -569,8 +616,7 @@ dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
   if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return DD_BAD_ARG;
   const Layout L = make_layout(q, op);
   if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
-  if (variant < DD_ADJ_PER_PIXEL || variant > DD_ADJ_TILE8) variant = g_adjoint_variant;
-  return run_tiles(q, op, variant, input, output, geo_ptrs(geo, L),
+  return run_tiles(q, op, adjoint_variant(variant, q), input, output, geo_ptrs(geo, L),
                    static_cast<char*>(ws), L, static_cast<hipStream_t>(stream));
 }
 

@@ -333,6 +333,129 @@ __device__ __forceinline__ void pixel_one_view(
   }
 }
 
+// Contributions of view v to the two consecutive pixels i0, i0 + 1 of one
+// line (i0 + 1 only when has1): one first-bin search for i0, then one walk
+// over the bins that reach either pixel, each sinogram tile loaded once.  A
+// bin contributes to pixel i exactly when its edge cells bracket i
+// (p1.c <= i <= p2.c), which is the set and order of bins pixel_one_view
+// walks for i, so each pixel's sum is the same bit for bit.
+template <int BT, int WM>
+__device__ __forceinline__ void pixel_pair_view(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const LineGeo& lg, float pl, const float* __restrict__ sino, int i0, bool has1, int N,
+    int b0, float gw0, float gw1, float (&acc0)[BT], float (&acc1)[BT]) {
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  const int i1 = i0 + 1, ilast = has1 ? i1 : i0;
+  int k = min(max(guess_bin_fast(g, vg, lg.a0 + (i0 + 0.5) * static_cast<double>(pl)), 0),
+              g.D - 1);
+  EdgePos p1 = edge_pos(ev[k], lg.a0, lg.inv_p, N);
+  while (k > 0 && p1.c >= i0) p1 = edge_pos(ev[--k], lg.a0, lg.inv_p, N);
+  EdgePos p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  while (p2.c < i0) {
+    if (++k >= g.D) return;
+    p1 = p2;
+    p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  }
+  const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+  while (p1.c <= ilast) {
+    float q[BT];
+    load_tile<BT>(sv + static_cast<size_t>(k) * g.Bp, q);
+    if (p1.c <= i0 && i0 <= p2.c) {
+      const float w = overlap_in_pixel(p1, p2, i0) * gw0;
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc0[b] = fmaf(w, q[b], acc0[b]);
+    }
+    if (has1 && p1.c <= i1 && i1 <= p2.c) {
+      const float w = overlap_in_pixel(p1, p2, i1) * gw1;
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc1[b] = fmaf(w, q[b], acc1[b]);
+    }
+    if (++k >= g.D) break;
+    p1 = p2;
+    if (p1.c > ilast) break;
+    p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  }
+}
+
+// sinogram -> image for the 2 x 2 pixel block at (r0, c0), all views: per
+// view the block is two lines of two consecutive pixels whichever the axis
+// (image rows c0, c0 + 1 on rows r0, r0 + 1; or columns, read bottom-up).
+// Equal bit for bit to pixels_body on each of the four pixels.
+template <int BT, int WM>
+__device__ __forceinline__ void quads_body(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out, int r0, int c0, int b0) {
+  const bool hr = r0 + 1 < g.H, hc = c0 + 1 < g.W;
+  float xc[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, yc[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+  if constexpr (WM == W_BPW) {
+#pragma unroll
+    for (int dr = 0; dr < 2; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < 2; ++dc) pixel_centre(g, 1, r0 + dr, c0 + dc, &xc[dr][dc], &yc[dr][dc]);
+  }
+  float acc[2][2][BT];  // [row offset][column offset][instance]
+#pragma unroll
+  for (int dr = 0; dr < 2; ++dr)
+#pragma unroll
+    for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc[dr][dc][b] = 0.0f;
+  const float kbp = bp_const(g);
+  for (int v = 0; v < g.V; ++v) {
+    const ViewGeo vg = views[v];
+    const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
+    const float* __restrict__ pv = pint + static_cast<size_t>(v) * g.Lmax;
+    if (vg.xaxis) {
+      // lines = image rows r0, r0 + 1; pixels c0, c0 + 1
+#pragma unroll
+      for (int dr = 0; dr < 2; ++dr) {
+        if (dr == 1 && !hr) break;
+        const int l = r0 + dr;
+        const float pl = pv[l];
+        float gw0 = (WM == W_FP) ? pl : kbp, gw1 = gw0;
+        if constexpr (WM == W_BPW) {
+          gw0 *= fbp_weight(g, vg, xc[dr][0], yc[dr][0]);
+          gw1 *= fbp_weight(g, vg, xc[dr][1], yc[dr][1]);
+        }
+        pixel_pair_view<BT, WM>(g, vg, v, edges, lv[l], pl, sino, c0, hc, g.W, b0, gw0, gw1,
+                                acc[dr][0], acc[dr][1]);
+      }
+    } else {
+      // lines = image columns c0, c0 + 1; pixels i = H - 1 - r: the pair
+      // (i0, i0 + 1) is rows (r0 + 1, r0), or row r0 alone on the last row
+#pragma unroll
+      for (int dc = 0; dc < 2; ++dc) {
+        if (dc == 1 && !hc) break;
+        const int l = c0 + dc;
+        const float pl = pv[l];
+        float gw0 = (WM == W_FP) ? pl : kbp, gw1 = gw0;
+        if (hr) {
+          if constexpr (WM == W_BPW) {
+            gw0 *= fbp_weight(g, vg, xc[1][dc], yc[1][dc]);
+            gw1 *= fbp_weight(g, vg, xc[0][dc], yc[0][dc]);
+          }
+          pixel_pair_view<BT, WM>(g, vg, v, edges, lv[l], pl, sino, g.H - 2 - r0, true, g.H,
+                                  b0, gw0, gw1, acc[1][dc], acc[0][dc]);
+        } else {
+          if constexpr (WM == W_BPW) gw0 *= fbp_weight(g, vg, xc[0][dc], yc[0][dc]);
+          pixel_pair_view<BT, WM>(g, vg, v, edges, lv[l], pl, sino, g.H - 1 - r0, false, g.H,
+                                  b0, gw0, gw1, acc[0][dc], acc[1][dc]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int dr = 0; dr < 2; ++dr)
+#pragma unroll
+    for (int dc = 0; dc < 2; ++dc)
+      if ((dr == 0 || hr) && (dc == 0 || hc))
+#pragma unroll
+        for (int b = 0; b < BT; ++b)
+          if (b0 + b < g.B)
+            out[(static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc] = acc[dr][dc][b];
+}
+
 // sinogram -> image for one pixel, all views, each view found independently.
 template <int BT, int WM>
 __device__ __forceinline__ void pixels_body(

@@ -48,8 +48,8 @@ struct BatchInst {
   float* part;         // partial bin sums per line chunk [kSplit][B][V][D]
   long long img_off;   // floats, into the packed image buffer
   long long sino_off;  // floats, into the packed sinogram buffer
-  int item0[5];        // first work item of each phase (see Phase)
-  int nitems[5];
+  int item0[6];        // first work item of each phase (see Phase)
+  int nitems[6];
 };
 
 enum Phase {
@@ -58,7 +58,8 @@ enum Phase {
   kLayoutSino = 2,
   kGatherPixels = 3,
   kReduceBins = 4,
-  kPhases = 5
+  kGatherQuads = 5,  // sinogram -> image with one thread per 2 x 2 pixels
+  kPhases = 6
 };
 
 constexpr int kBT = 1;  // one geometry per instance
@@ -201,6 +202,26 @@ __global__ __launch_bounds__(kThreads) void k_batch_pixels(
   }
 }
 
+// Quad items: (instance, 32 x 32 pixel tile), one thread per 2 x 2 pixels.
+template <int WM>
+__global__ __launch_bounds__(kThreads) void k_batch_quads(
+    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(inst, n_inst, kGatherQuads, item);
+    const BatchInst& d = inst[i];
+    const int local = item - d.item0[kGatherQuads];
+    const int tx = (d.g.W + 2 * kTile - 1) / (2 * kTile);
+    const int ty = (d.g.H + 2 * kTile - 1) / (2 * kTile);
+    const int t = local % (tx * ty);
+    const int b0 = (local / (tx * ty)) * kBT;
+    const int c0 = (t % tx) * (2 * kTile) + 2 * (threadIdx.x % kTile);
+    const int r0 = (t / tx) * (2 * kTile) + 2 * (threadIdx.x / kTile);
+    if (c0 < d.g.W && r0 < d.g.H)
+      quads_body<kBT, WM>(d.g, d.views, d.edges, d.lines, d.pint, d.sino, out + d.img_off, r0,
+                          c0, b0);
+  }
+}
+
 }  // namespace
 }  // namespace dd
 
@@ -212,9 +233,9 @@ struct dd2d_batch {
   std::vector<int> order;        // instance order (heaviest first)
   size_t desc_off = 0, angles_off = 0, bytes = 0;
   size_t total_views = 0;
-  int total[kPhases] = {0, 0, 0, 0, 0};
+  int total[kPhases] = {0, 0, 0, 0, 0, 0};
   int grid = 0;                  // persistent workgroups (0 = automatic)
-  int variant = DD_ADJ_PER_PIXEL; // sinogram -> image kernel (per pixel or 16x16 tiles)
+  int variant = DD_ADJ_QUAD;     // sinogram -> image kernel (per pixel, 16x16 tiles, 2x2 quads)
   int cus = 0;
   char* dev = nullptr;           // device memory given to setup()
 };
@@ -263,7 +284,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
   b->desc_off = take(&off, sizeof(BatchInst) * n);
   b->angles_off = take(&off, sizeof(float*) * n);
   b->inst.resize(n);
-  int acc[kPhases] = {0, 0, 0, 0, 0};
+  int acc[kPhases] = {0, 0, 0, 0, 0, 0};
   for (int j = 0; j < n; ++j) {
     const auto& it = items[b->order[j]];
     BatchInst d{};
@@ -303,6 +324,8 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     d.nitems[kLayoutImage] = g.H * tiles;
     d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kThreads - 1) / kThreads) * tiles;
     d.nitems[kReduceBins] = g.V * ((g.D + kThreads - 1) / kThreads) * it.batch;
+    d.nitems[kGatherQuads] =
+        ((g.W + 2 * kTile - 1) / (2 * kTile)) * ((g.H + 2 * kTile - 1) / (2 * kTile)) * tiles;
     d.nitems[kLayoutSino] = g.V * tiles;
     d.nitems[kGatherPixels] = ((g.W + kTile - 1) / kTile) * ((g.H + kTile - 1) / kTile) * tiles;
     for (int ph = 0; ph < kPhases; ++ph) {
@@ -328,7 +351,9 @@ void dd2d_batch_set_grid(dd2d_batch* b, int workgroups) {
 }
 
 void dd2d_batch_set_variant(dd2d_batch* b, int variant) {
-  if (b) b->variant = (variant == DD_ADJ_TILE16) ? DD_ADJ_TILE16 : DD_ADJ_PER_PIXEL;
+  if (b)
+    b->variant = (variant == DD_ADJ_TILE16 || variant == DD_ADJ_QUAD) ? variant
+                                                                       : DD_ADJ_PER_PIXEL;
 }
 
 dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* stream) {
@@ -410,8 +435,21 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
       hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutSino])),
                          dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutSino),
                          b->total[kLayoutSino], input);
-    const int tot = b->total[kGatherPixels];
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
+    if (b->variant == DD_ADJ_QUAD) {
+      const int qt = b->total[kGatherQuads];
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL((k_batch_quads<W_FP>), dim3(grid_for(b, qt)), dim3(kThreads), 0, st,
+                           inst, n, qt, output);
+      else if (op == DD_BPROJ)
+        hipLaunchKernelGGL((k_batch_quads<W_BP>), dim3(grid_for(b, qt)), dim3(kThreads), 0, st,
+                           inst, n, qt, output);
+      else
+        hipLaunchKernelGGL((k_batch_quads<W_BPW>), dim3(grid_for(b, qt)), dim3(kThreads), 0, st,
+                           inst, n, qt, output);
+      return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+    }
+    const int tot = b->total[kGatherPixels];
     if (op == DD_PROJ_T)
       hipLaunchKernelGGL((k_batch_pixels<W_FP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
                          inst, n, tot, output, b->variant != DD_ADJ_TILE16);

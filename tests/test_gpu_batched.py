@@ -35,21 +35,23 @@ def test_plan_ops_equal_plain_ops(scan):
         assert torch.equal(BP.backprojection_w(y), P.backprojection_w(y, ang))
 
 
+@pytest.mark.parametrize("n", [100, 99])
 @pytest.mark.parametrize("scan", ["para", "flat", "arc"])
-def test_adjoint_kernel_variants_agree_bitwise(scan):
-    """Per-pixel and LDS-tiled (16x16, 8x8) sinogram -> image kernels sum
-    the same overlaps in the same order, so they agree bit for bit; sizes
-    that are not tile multiples exercise the partial tiles."""
+def test_adjoint_kernel_variants_agree_bitwise(scan, n):
+    """Per-pixel, LDS-tiled (16x16, 8x8) and 2x2-quad sinogram -> image
+    kernels sum the same overlaps in the same order, so they agree bit for
+    bit; sizes that are not tile multiples (odd, too) exercise the partial
+    tiles."""
     from pridediff_amd import load_radontorch
     from pridediff_amd.batched import BatchedProjector2d
     rt = load_radontorch()
-    P, ang, x, y = _setup(n=100, d=151, v=90, b=3, scan=scan)
+    P, ang, x, y = _setup(n=n, d=151, v=90, b=3, scan=scan)
     outs = {}
-    for var in (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8):
+    for var in (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD):
         BP = BatchedProjector2d(P, ang, x.shape[0], variant=var)
         with torch.no_grad():
             outs[var] = (BP.projection_t(y), BP.backprojection(y), BP.backprojection_w(y))
-    for var in (rt.ADJ_TILE16, rt.ADJ_TILE8):
+    for var in (rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD):
         for a, b in zip(outs[rt.ADJ_PER_PIXEL], outs[var]):
             assert torch.equal(a, b), (scan, var, (a - b).abs().max().item())
 
@@ -77,7 +79,7 @@ def _uneven(n_inst=8, seed=50):
     rng = np.random.default_rng(seed)
     projs, angs, imgs = [], [], []
     for i in range(n_inst):
-        n = int(rng.choice([64, 96, 128]))
+        n = int(rng.choice([64, 96, 128, 75]))
         v = int(rng.integers(24, 91))
         d = int(np.ceil(1.44 * n))
         projs.append(projector2d(n, d, 0.7 * 4, 1.2858 * 4, 595.0, 1085.6, scan_type="arc").cuda())
@@ -95,9 +97,13 @@ def test_uneven_batch_equals_per_instance_ops():
         for P, a, im, s in zip(projs, angs, imgs, sinos):
             assert torch.equal(s, P.projection(im[None, None], a)[0, 0])
         ys = [torch.rand_like(s) for s in sinos]
-        backs = ub.unpack_images(ub.projection_t(ub.pack_sinos(ys)))
-        for P, a, y, b in zip(projs, angs, ys, backs):
-            assert torch.equal(b, P.projection_t(y[None, None], a)[0, 0])
+        from pridediff_amd import load_radontorch
+        rt = load_radontorch()
+        for var in (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD):
+            ub.plan.set_variant(var)
+            backs = ub.unpack_images(ub.projection_t(ub.pack_sinos(ys)))
+            for P, a, y, b in zip(projs, angs, ys, backs):
+                assert torch.equal(b, P.projection_t(y[None, None], a)[0, 0]), var
 
 
 def test_uneven_graph_replay_and_autotune():
