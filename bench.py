@@ -122,11 +122,23 @@ def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
             xi = im[None, None].detach().clone().requires_grad_(True)
             data_consistency_grad(lambda t: P.projection(t, a), xi, yy[None, None])
     t_p = timed(torch, plain, 2, 1)
+    from pridediff_amd import load_radontorch
+    rt = load_radontorch()
+    rt.profile_enable(True)
+    rt.profile_read(0), rt.profile_read(1)
+    for _ in range(3):
+        xg = x.detach().clone().requires_grad_(True)
+        data_consistency_grad(ub.projection, xg, y)
+    torch.cuda.synchronize()
+    (fp_ms, fp_n), (bp_ms, bp_n) = rt.profile_read(0), rt.profile_read(1)
+    rt.profile_enable(False)
     shapes = ub.shapes
     b_alg = sum(8.0 * v * (h * w + d) for h, w, v, d in shapes)
     out = {"instances": 64, "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
            "speedup": t_p / t_b, "roofline_frac": b_alg / t_b / HBM_PEAK,
-           "algorithmic_GB": b_alg / 1e9, "grid": tune.get("grid")}
+           "algorithmic_GB": b_alg / 1e9, "grid": tune.get("grid"),
+           "variant": tune.get("variant"),
+           "forward_ms": fp_ms / max(fp_n, 1), "adjoint_ms": bp_ms / max(bp_n, 1)}
     if check:
         from oracle import oracle as O
         j = min(range(64), key=lambda i: shapes[i][0] * shapes[i][2])
