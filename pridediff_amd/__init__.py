@@ -8,7 +8,11 @@ Layers, top down:
   libdd.so                                 HIP kernels for gfx950 behind a
       C ABI (include/dd.h, include/dd3d.h).
   pridediff_amd.batched                    one-launch batched FP/adjoint for
-      many instances, graph-captured, with an autotuned tile shape.
+      many instances (uniform or uneven), graph-captured steps;
+  pridediff_amd.autotune                   tile / kernel-variant / grid tuner;
+  pridediff_amd.fbp                        FBP as one GEMM + weighted
+      backprojection, PrideDiff's data-consistency step;
+  pridediff_amd.views                      view-resolved backprojection.
 
 ``install_dropin()`` registers the modules under the reference's import names
 (``radontorch``, ``deeprecon``) so unmodified callers pick them up.

@@ -118,6 +118,17 @@ class UnevenBatch2d(torch.nn.Module):
     def projection_t(self, flat_sinos):
         return _PlannedOp.apply(flat_sinos, self.plan, rt.PROJ_T, rt.PROJ)
 
+    def backprojection(self, flat_sinos):
+        return _PlannedOp.apply(flat_sinos, self.plan, rt.BPROJ, rt.BPROJ_T)
+
+    def backprojection_t(self, flat_images):
+        return _PlannedOp.apply(flat_images, self.plan, rt.BPROJ_T, rt.BPROJ)
+
+    def backprojection_w(self, flat_sinos):
+        if self.scan_type == "para":  # the reference maps it to BackProj (projector.py:49)
+            return self.backprojection(flat_sinos)
+        return _PlannedOp.apply(flat_sinos, self.plan, rt.BPROJ_W, rt.BPROJ_TW)
+
     def forward(self, flat_images):
         return self.projection(flat_images)
 

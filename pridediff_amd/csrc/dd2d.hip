@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 
@@ -31,23 +32,23 @@ namespace {
 
 using namespace body2d;
 
-int g_forced_tile = 0;
+std::atomic<int> g_forced_tile{0};
 // sinogram -> image kernel of dd2d_run and of plans without a variant:
 // -1 = automatic: 2 x 2 quads for up to 4 instances per launch when they
 // still give at least 4 workgroups per compute unit, per pixel otherwise.
-int g_adjoint_variant = -1;
+std::atomic<int> g_adjoint_variant{-1};
 
+// Compute units of the first device the library runs on (thread-safe static;
+// every device of one process is the same part here).
 int compute_units() {
-  static int cus = 0;  // first call wins; every device of a process is an MI355X here
-  if (cus == 0) {
+  static const int cus = [] {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
         n > 0)
-      cus = n;
-    else
-      cus = 256;
-  }
+      return n;
+    return 256;
+  }();
   return cus;
 }
 
@@ -269,12 +270,11 @@ struct Layout {
   size_t buf0, buf1, part, data_total;                // data buffers
 };
 
-
-
 constexpr size_t kAlign = 256;
 
 int pick_tile(int batch) {
-  if (g_forced_tile > 0) return g_forced_tile;
+  const int forced = g_forced_tile.load();
+  if (forced > 0) return forced;
   int t = 1;
   while (t < batch && t < 16) t *= 2;
   return t;

@@ -104,6 +104,11 @@ def test_uneven_batch_equals_per_instance_ops():
             backs = ub.unpack_images(ub.projection_t(ub.pack_sinos(ys)))
             for P, a, y, b in zip(projs, angs, ys, backs):
                 assert torch.equal(b, P.projection_t(y[None, None], a)[0, 0]), var
+        bw = ub.unpack_images(ub.backprojection_w(ub.pack_sinos(ys)))
+        bt = ub.unpack_sinos(ub.backprojection_t(ub.pack_images(imgs)))
+        for P, a, y, im, b1, b2 in zip(projs, angs, ys, imgs, bw, bt):
+            assert torch.equal(b1, P.backprojection_w(y[None, None], a)[0, 0])
+            assert torch.equal(b2, P.backprojection_t(im[None, None], a)[0, 0])
 
 
 def test_uneven_graph_replay_and_autotune():
