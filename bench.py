@@ -51,10 +51,11 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--workload", default="fanarc",
-                    choices=["fanarc", "cone_flat", "helical", "fidelity"],
+                    choices=["fanarc", "uneven", "cone_flat", "helical", "fidelity"],
                     help="fanarc: 512^2 fan-arc batch 16 (headline); cone_flat: 256^3 cone-flat "
                          "batch 2; helical: 512x512x64 helical batch 4; fidelity: PrideDiff's "
-                         "data-consistency step x - w FBP(A x - y) on the fan-arc batch")
+                         "data-consistency step x - w FBP(A x - y) on the fan-arc batch; "
+                         "uneven: 64 fan-arc instances of uneven size in one persistent launch")
     return ap.parse_args()
 
 
@@ -135,6 +136,7 @@ def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
     shapes = ub.shapes
     b_alg = sum(8.0 * v * (h * w + d) for h, w, v, d in shapes)
     out = {"instances": 64, "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
+           "detector_samples": sum(v * d for _, _, v, d in shapes),
            "speedup": t_p / t_b, "roofline_frac": b_alg / t_b / HBM_PEAK,
            "algorithmic_GB": b_alg / 1e9, "grid": tune.get("grid"),
            "variant": tune.get("variant"),
@@ -397,6 +399,24 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.workload == "uneven":
+        dev = torch.device("cuda:0")
+        c2 = uneven_c2(torch, dev, args.steps, args.warmup, not args.no_accuracy, not args.no_tune)
+        samples = c2.pop("detector_samples")
+        print(json.dumps({
+            "metric": "FP+BP detector samples/s & %HBM roofline, 64 uneven fan-arc instances",
+            "value": samples / (c2["batched_ms"] * 1e-3), "unit": "detector samples/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": c2["batched_ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded ellipse phantoms)",
+            "config": {"workload": "64 fan-arc instances, N in {256,384,512}, V in [96,720], "
+                                   "D = ceil(1.44 N), one persistent launch per op"},
+            "roofline": {"bound": "hbm", "achieved": c2["algorithmic_GB"] / c2["batched_ms"] * 1e3,
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": c2["roofline_frac"],
+                         "traffic": None, "kernel": "step"},
+            "uneven_c2": c2, "cpu_baseline": None}), flush=True)
+        return
     if args.workload == "fidelity":
         print(json.dumps(bench_fidelity(torch, args.steps, args.warmup, not args.no_accuracy)),
               flush=True)
@@ -485,6 +505,8 @@ def main():
 
     c2 = None if args.no_c2 else uneven_c2(torch, dev, max(3, args.steps // 2), 2,
                                            not args.no_accuracy, not args.no_tune)
+    if c2 is not None:
+        c2.pop("detector_samples")
     cpu = None if args.no_cpu else cpu_baseline(ang_np, x_np, y_np)
     samples = B * V * D
     out = {
