@@ -190,3 +190,24 @@ def test_errors_are_reported(rt):
         rt.ProjParaDis2d(torch.zeros(1, 1, 32, 32, dtype=torch.float64).cuda(), opt, ang)
     with pytest.raises(RuntimeError, match="sinogram"):
         rt.ProjTransParaDis2d(torch.zeros(1, 1, 5, 40).cuda(), opt, ang)
+
+
+def test_parallel_c0_workload(rt):
+    """The reference's own CPU case: parallel beam, 256x256 image of 10 seeded
+    ellipses, pixel 1.0 mm, 180 views evenly over [0, pi), 367 bins of
+    1.0 mm, batch 1; forward projection and backprojection (and ProjTrans)
+    against the FP64 oracle and, where oracle/_ref is built, the reference's
+    own code."""
+    g = O.Geom2D(256, 256, 367, 1.0, 1.0, 1.0, 1.0, 1.0)
+    ang = (np.arange(180, dtype=np.float32) * np.float32(np.pi / 180)).astype(np.float32)
+    x = ellipses2d(256, 1, count=10, seed=0)
+    sino = O.op2d("para", "proj", x, ang, g, "f64").astype(np.float32)
+    for op, inp in (("proj", x), ("bproj", sino), ("proj_t", sino)):
+        got = gpu_op(rt, "para", op, inp, ang, g)
+        want = O.op2d("para", op, inp, ang, g, "f64")
+        assert report(f"C0 para {op} vs FP64", got, want) <= 1e-5
+        if O.ref_available():
+            ref = O.op2d("para", op, inp, ang, g, "ref")
+            floor = O.rel_l2(ref, want)
+            r = report(f"C0 para {op} vs reference (its floor {floor:.2e})", got, ref)
+            assert r <= max(1e-5, 2 * floor)
