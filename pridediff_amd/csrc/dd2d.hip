@@ -59,6 +59,14 @@ int adjoint_variant(int requested, const Geo2& g) {
   return (g.BT <= 4 && quad_groups >= 4LL * compute_units()) ? DD_ADJ_QUAD : DD_ADJ_PER_PIXEL;
 }
 
+// Pixel workgroup width of the per-pixel sinogram -> image gather (256
+// threads): 64 x 4 pixels when 8+ instances share a launch (a wave is one
+// image row, one line of every view on the row axis), 16 x 16 otherwise.
+template <int BT>
+constexpr int pix_tile_w() {
+  return BT >= 8 ? 64 : 16;
+}
+
 // ---------------------------------------------------------------------------
 // one-call kernels: thin grids over the shared bodies
 // ---------------------------------------------------------------------------
@@ -154,8 +162,9 @@ __global__ __launch_bounds__(kThreads) void k_pixels(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
     const float* __restrict__ sino, float* __restrict__ out) {
-  const int c = blockIdx.x * kTile + (threadIdx.x % kTile);
-  const int r = blockIdx.y * kTile + (threadIdx.x / kTile);
+  constexpr int TW = pix_tile_w<BT>(), TH = kThreads / TW;
+  const int c = blockIdx.x * TW + (threadIdx.x % TW);
+  const int r = blockIdx.y * TH + (threadIdx.x / TW);
   if (c >= g.W || r >= g.H) return;
   pixels_body<BT, WM>(g, views, edges, lines, pint, sino, out, r, c, blockIdx.z * BT);
 }
@@ -440,7 +449,8 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
         hipLaunchKernelGGL((k_pixels_tile<BT, W_BPW, 8>), grid, dim3(64), 0, st, g,
                            G.views, G.edges, G.lines, G.pint, buf0, output);
     } else {
-      dim3 grid((g.W + kTile - 1) / kTile, (g.H + kTile - 1) / kTile, ntile);
+      constexpr int TW = pix_tile_w<BT>(), TH = kThreads / TW;
+      dim3 grid((g.W + TW - 1) / TW, (g.H + TH - 1) / TH, ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
                            G.views, G.edges, G.lines, G.pint, buf0, output);

@@ -115,11 +115,15 @@ __global__ __launch_bounds__(kThreads) void k_batch_layout(
   }
 }
 
-// Gather-bins items: (view, bin block, instance, line chunk), chunk fastest;
-// every instance's item count and first item are multiples of kSplit, so with
-// a grid that is a multiple of 8 the chunk follows the XCD.
+// Gather-bins items: (64-bin block, view, instance, line chunk), chunk
+// fastest, one wavefront per item (a 64-thread workgroup), so instances whose
+// bin count is not a multiple of 256 keep their lanes busy.  Every instance's
+// item count and first item are multiples of kSplit, so with a grid that is a
+// multiple of 8 the chunk follows the XCD.
+constexpr int kBinsWave = 64;
+
 template <int WM>
-__global__ __launch_bounds__(kThreads) void k_batch_bins(
+__global__ __launch_bounds__(kBinsWave) void k_batch_bins(
     const BatchInst* __restrict__ inst, int n_inst, int total) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int i = find_inst(inst, n_inst, kGatherBins, item);
@@ -127,8 +131,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_bins(
     const int local = item - d.item0[kGatherBins];
     const int chunk = local % kSplit;
     int rest = local / kSplit;
-    const int kb_n = (d.g.D + kThreads - 1) / kThreads;
-    const int k = (rest % kb_n) * kThreads + threadIdx.x;
+    const int kb_n = (d.g.D + kBinsWave - 1) / kBinsWave;
+    const int k = (rest % kb_n) * kBinsWave + threadIdx.x;
     rest /= kb_n;
     const int v = rest % d.g.V;
     const int b0 = (rest / d.g.V) * kBT;
@@ -322,7 +326,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     d.sino_off = it.sino_offset;
     const int tiles = it.batch;  // kBT == 1
     d.nitems[kLayoutImage] = g.H * tiles;
-    d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kThreads - 1) / kThreads) * tiles;
+    d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kBinsWave - 1) / kBinsWave) * tiles;
     d.nitems[kReduceBins] = g.V * ((g.D + kThreads - 1) / kThreads) * it.batch;
     d.nitems[kGatherQuads] =
         ((g.W + 2 * kTile - 1) / (2 * kTile)) * ((g.H + 2 * kTile - 1) / (2 * kTile)) * tiles;
@@ -409,20 +413,22 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
                        dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutImage),
                        b->total[kLayoutImage], input);
     const int tot = b->total[kGatherBins], rtot = b->total[kReduceBins];
+    // one-wave workgroups: four times the workgroups of the 256-thread phases
+    const int bgrid = std::max(1, std::min(4 * grid_for(b, tot), tot));
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ) {
-      hipLaunchKernelGGL((k_batch_bins<W_FP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
-                         inst, n, tot);
+      hipLaunchKernelGGL((k_batch_bins<W_FP>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
+                         tot);
       hipLaunchKernelGGL((k_batch_reduce<W_FP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
                          st, inst, n, rtot, output);
     } else if (op == DD_BPROJ_T) {
-      hipLaunchKernelGGL((k_batch_bins<W_BP>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
-                         inst, n, tot);
+      hipLaunchKernelGGL((k_batch_bins<W_BP>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
+                         tot);
       hipLaunchKernelGGL((k_batch_reduce<W_BP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
                          st, inst, n, rtot, output);
     } else {
-      hipLaunchKernelGGL((k_batch_bins<W_BPW>), dim3(grid_for(b, tot)), dim3(kThreads), 0, st,
-                         inst, n, tot);
+      hipLaunchKernelGGL((k_batch_bins<W_BPW>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
+                         tot);
       hipLaunchKernelGGL((k_batch_reduce<W_BPW>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
                          st, inst, n, rtot, output);
     }
