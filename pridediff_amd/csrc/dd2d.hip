@@ -164,7 +164,10 @@ __global__ __launch_bounds__(kThreads) void k_pixels(
     const float* __restrict__ sino, float* __restrict__ out) {
   constexpr int TW = pix_tile_w<BT>(), TH = kThreads / TW;
   const int c = blockIdx.x * TW + (threadIdx.x % TW);
-  const int r = blockIdx.y * TH + (threadIdx.x / TW);
+  // with 64-wide rows a wave is one image row: say so, for scalar row loads
+  const int ty = TW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x / TW)
+                          : static_cast<int>(threadIdx.x / TW);
+  const int r = blockIdx.y * TH + ty;
   if (c >= g.W || r >= g.H) return;
   pixels_body<BT, WM>(g, views, edges, lines, pint, sino, out, r, c, blockIdx.z * BT);
 }

@@ -291,18 +291,22 @@ __device__ __forceinline__ void pixel_one_view(
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
     const float* __restrict__ sino, int r, int c, int b0, float xcw, float ycw,
     float (&acc)[BT]) {
-  int l, i, N;
+  // line geometry loaded in the branch, so a caller whose row r is
+  // wave-uniform gets scalar loads on row-axis views
+  int i, N;
+  LineGeo lg;
+  float pl;
   if (vg.xaxis) {
-    l = r;
     i = c;
     N = g.W;
+    lg = lines[static_cast<size_t>(v) * g.Lmax + r];
+    pl = pint[static_cast<size_t>(v) * g.Lmax + r];
   } else {
-    l = c;
     i = g.H - 1 - r;
     N = g.H;
+    lg = lines[static_cast<size_t>(v) * g.Lmax + c];
+    pl = pint[static_cast<size_t>(v) * g.Lmax + c];
   }
-  const LineGeo lg = lines[static_cast<size_t>(v) * g.Lmax + l];
-  const float pl = pint[static_cast<size_t>(v) * g.Lmax + l];
   const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
   // k = the last bin whose lower edge lies before pixel i (or 0): unique,
   // whatever the guess.  The positions of edges k and k + 1 found on the way
