@@ -381,9 +381,28 @@ __device__ __forceinline__ ZAffine z_affine(const ViewGeo3& vg, const LineGeo3& 
   return {(u - lg.z0) * lg.inv_pz, (m * vg.rowp) * lg.inv_pz};
 }
 
-__device__ __forceinline__ EdgePos row_pos(const ZAffine& z, int r, int Dz) {
-  return edge_pos_t(fma(static_cast<double>(r), z.B, z.A), Dz);
-}
+// Row edges r0 + j (j = 0 .. a few) of one column on one line's slice grid:
+// t = A + (r0 + j) B, kept as floor(t_r0) plus an FP32 offset (|offset| is
+// a few slices, so FP32 holds it to ~1e-6 of a slice), clamped to
+// [-2, Dz + 2] like edge_pos_t.
+struct RowEdges {
+  int base;
+  float dt, b, lo, hi;
+  __device__ RowEdges(const ZAffine& z, int r0, int Dz) {
+    const double t0 = fma(static_cast<double>(r0), z.B, z.A);
+    const double tb = floor(fmin(fmax(t0, -1.0e6), 1.0e6));
+    base = static_cast<int>(tb);
+    dt = static_cast<float>(fmin(fmax(t0, -1.0e6), 1.0e6) - tb);
+    b = static_cast<float>(z.B);
+    lo = static_cast<float>(-2 - base);
+    hi = static_cast<float>(Dz + 2 - base);
+  }
+  __device__ EdgePos pos(int j) const {
+    const float t = fminf(fmaxf(fmaf(static_cast<float>(j), b, dt), lo), hi);
+    const float fl = floorf(t);
+    return {base + static_cast<int>(fl), t - fl};
+  }
+};
 
 // Rows (forward) and slices (adjoint) one thread carries in registers.
 #ifndef DD3D_FWD_ROWS
@@ -446,15 +465,17 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
     const int cs = max(p1.c, 0), cl = min(p2.c, N - 1);
     if (cs > cl) continue;
     const ZAffine z = z_affine(vg, lg, m);
-    EdgePos qa = row_pos(z, r0, g.Dz);
-    const EdgePos qend = row_pos(z, r0 + nr, g.Dz);
+    // the chunk's row edges t_r0 .. t_r0+nr in FP32 relative to floor(t_r0)
+    const RowEdges re(z, r0, g.Dz);
+    EdgePos qa = re.pos(0);
+    const EdgePos qend = re.pos(nr);
     if (qend.c < 0 || qa.c >= g.Dz) continue;  // the chunk's rows miss the slab
     const float wl = (WM == W3_FP) ? lg.p * lg.pzf : vg.kbp;
     const float* __restrict__ line = src + static_cast<size_t>(l) * g.Dz * N * g.Bp;
 #pragma unroll
     for (int j = 0; j < S; ++j) {
       if (j < nr) {
-        const EdgePos qb = (j + 1 == nr) ? qend : row_pos(z, r0 + j + 1, g.Dz);
+        const EdgePos qb = (j + 1 == nr) ? qend : re.pos(j + 1);
         const int ss = max(qa.c, 0), se = min(qb.c, g.Dz - 1);
         for (int sl = ss; sl <= se; ++sl) {
           const float wz = overlap_in_pixel(qa, qb, sl) * wl;
