@@ -248,8 +248,9 @@ __device__ __forceinline__ void bins_accumulate(
 // 3b. sinogram -> image: one thread per pixel, BT instances, all views
 // ---------------------------------------------------------------------------
 // First walk-ordered bin worth testing for an axis point a: the inverse of
-// the edge map, closed form per geometry, in FP32.  Only a starting guess;
-// the exact bin is then fixed against the edge table with edge_pos().
+// the edge map, closed form per geometry, in FP32 with hardware reciprocals
+// (v_rcp_f32) instead of IEEE divisions.  Only a starting guess; the exact
+// bin is then fixed against the edge table with edge_pos().
 __device__ __forceinline__ int guess_bin_fast(const Geo2& g, const ViewGeo& vg,
                                               double a) {
   float kf;
@@ -267,16 +268,17 @@ __device__ __forceinline__ int guess_bin_fast(const Geo2& g, const ViewGeo& vg,
     const float d =
         t * (0.9998660f +
              t2 * (-0.3302995f + t2 * (0.1801410f + t2 * (-0.0851330f + t2 * 0.0208351f))));
-    kf = g.D * 0.5f + (vg.flip ? -d : d) / g.det;
+    kf = g.D * 0.5f + (vg.flip ? -d : d) * __builtin_amdgcn_rcpf(g.det);
   } else if (g.scan == DD_FAN_FLAT) {
     const float R = g.sid, af = static_cast<float>(a);
     const float sf = static_cast<float>(vg.s), cf = static_cast<float>(vg.c);
-    const float u = vg.xaxis ? R * af * cf / (R + sf * af) : R * af * sf / (R - cf * af);
+    const float u = vg.xaxis ? R * af * cf * __builtin_amdgcn_rcpf(R + sf * af)
+                             : R * af * sf * __builtin_amdgcn_rcpf(R - cf * af);
     const float vdet = g.sid / g.sdd * g.det, vbin = g.sid / g.sdd * g.bin;
-    kf = g.D * 0.5f + (vg.flip ? -(u - vbin) : (u - vbin)) / vdet;
+    kf = g.D * 0.5f + (vg.flip ? -(u - vbin) : (u - vbin)) * __builtin_amdgcn_rcpf(vdet);
   } else {
     const float u = static_cast<float>(vg.xaxis ? a * vg.c : a * vg.s);
-    kf = g.D * 0.5f + (vg.flip ? -(u - g.bin) : (u - g.bin)) / g.det;
+    kf = g.D * 0.5f + (vg.flip ? -(u - g.bin) : (u - g.bin)) * __builtin_amdgcn_rcpf(g.det);
   }
   kf = fminf(fmaxf(kf, -4.0f), g.D + 4.0f);
   return static_cast<int>(floorf(kf));

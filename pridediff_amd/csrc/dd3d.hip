@@ -419,6 +419,13 @@ template <int BT>
 constexpr int slices3() {
   return BT >= 4 ? DD3D_ADJ_SLICES / 2 : DD3D_ADJ_SLICES;
 }
+#ifndef DD3D_ADJ_WAVES
+#define DD3D_ADJ_WAVES 4
+#endif
+template <int BT>
+constexpr int adj_waves3() {
+  return DD3D_ADJ_WAVES;
+}
 
 // image -> sinogram: one thread per (view, column, chunk of S rows).  Per
 // line the column's x footprint and the z law are evaluated once and shared
@@ -522,9 +529,10 @@ __device__ __forceinline__ int guess_col_fast(const Geo3& g, const ViewGeo3& vg,
   if (g.scan == DD_CONE_FLAT) {
     const float R = g.sid, af = static_cast<float>(a);
     const float sf = static_cast<float>(vg.s), cf = static_cast<float>(vg.c);
-    const float u = vg.xaxis ? R * af * cf / (R + sf * af) : R * af * sf / (R - cf * af);
+    const float u = vg.xaxis ? R * af * cf * __builtin_amdgcn_rcpf(R + sf * af)
+                             : R * af * sf * __builtin_amdgcn_rcpf(R - cf * af);
     const float d = u - static_cast<float>(vg.cshp);
-    kf = g.C * 0.5f + (vg.flip ? -d : d) / static_cast<float>(vg.colp);
+    kf = g.C * 0.5f + (vg.flip ? -d : d) * __builtin_amdgcn_rcpf(static_cast<float>(vg.colp));
   } else {
     // d = gamma - theta_ref (mod pi) from tan d = (Y cos - X sin) / (X cos + Y sin),
     // atan by Abramowitz & Stegun 4.4.49 (|error| <= 1.2e-5 on [-1, 1])
@@ -537,14 +545,15 @@ __device__ __forceinline__ int guess_col_fast(const Geo3& g, const ViewGeo3& vg,
     const float d =
         t * (0.9998660f +
              t2 * (-0.3302995f + t2 * (0.1801410f + t2 * (-0.0851330f + t2 * 0.0208351f))));
-    kf = g.C * 0.5f + (vg.flip ? -d : d) / static_cast<float>(vg.colp);
+    kf = g.C * 0.5f + (vg.flip ? -d : d) * __builtin_amdgcn_rcpf(static_cast<float>(vg.colp));
   }
   kf = fminf(fmaxf(kf, 0.0f), g.C - 1.0f);
   return static_cast<int>(kf);
 }
 
 template <int BT, int WM>
-__global__ __launch_bounds__(kThreads3) void k_voxels3d(
+__global__ __launch_bounds__(kThreads3)
+__attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
     const float* __restrict__ sino, float* __restrict__ out) {
@@ -607,11 +616,12 @@ __global__ __launch_bounds__(kThreads3) void k_voxels3d(
       // slice edge t and L(u) interpolates the row prefix Q linearly.  u is
       // kept relative to the chunk's first row so float holds it exactly
       // enough.
-      const double u0 = (static_cast<double>(s0) - z.A) / z.B;
+      const double inv_b = 1.0 / z.B;
+      const double u0 = (static_cast<double>(s0) - z.A) * inv_b;
       const double rb = floor(u0);
       const int rbase = static_cast<int>(fmin(fmax(rb, -1.0e6), 1.0e6));
       const float du = static_cast<float>(u0 - rb);
-      const float invb = static_cast<float>(1.0 / z.B);
+      const float invb = static_cast<float>(inv_b);
       const float ulo = static_cast<float>(-rbase), uhi = static_cast<float>(g.R - rbase);
       if (uhi > 0.0f && fmaf(static_cast<float>(S), invb, du) > ulo) {
         const float wxb = wx * static_cast<float>(z.B);
