@@ -397,16 +397,44 @@ struct RowEdges {
     lo = static_cast<float>(-2 - base);
     hi = static_cast<float>(Dz + 2 - base);
   }
-  __device__ EdgePos pos(int j) const {
-    const float t = fminf(fmaxf(fmaf(static_cast<float>(j), b, dt), lo), hi);
-    const float fl = floorf(t);
-    return {base + static_cast<int>(fl), t - fl};
+  // edge j relative to base, clamped
+  __device__ float rel(int j) const {
+    return fminf(fmaxf(fmaf(static_cast<float>(j), b, dt), lo), hi);
   }
 };
+
+// Two neighbouring voxels of BT instances each (2 BT contiguous floats),
+// one vector load where the width allows.  p is only 4-byte aligned.
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+template <int BT>
+__device__ __forceinline__ void load_pair(const float* __restrict__ p, float (&x0)[BT],
+                                          float (&x1)[BT]) {
+  if constexpr (BT == 1) {
+    const f2u q = *reinterpret_cast<const f2u*>(p);
+    x0[0] = q.x;
+    x1[0] = q.y;
+  } else if constexpr (BT == 2) {
+    const f4u q = *reinterpret_cast<const f4u*>(p);
+    x0[0] = q.x;
+    x0[1] = q.y;
+    x1[0] = q.z;
+    x1[1] = q.w;
+  } else {
+    load_tile<BT>(p, x0);
+    load_tile<BT>(p + BT, x1);
+  }
+}
 
 // Rows (forward) and slices (adjoint) one thread carries in registers.
 #ifndef DD3D_FWD_ROWS
 #define DD3D_FWD_ROWS 4
+#endif
+#ifndef DD3D_FWD_PREFETCH
+#define DD3D_FWD_PREFETCH 0
+#endif
+#ifndef DD3D_FWD_PIPE
+#define DD3D_FWD_PIPE 0
 #endif
 #ifndef DD3D_ADJ_SLICES
 #define DD3D_ADJ_SLICES 16
@@ -429,7 +457,10 @@ constexpr int adj_waves3() {
 
 // image -> sinogram: one thread per (view, column, chunk of S rows).  Per
 // line the column's x footprint and the z law are evaluated once and shared
-// by the S rows; each row then walks its slices.
+// by the S rows; the slices the chunk reaches are then visited once each,
+// every slice's voxels loaded once and spread over the S rows by their z
+// overlaps (slice-major: a row-by-row walk reloads each slice for every
+// row that overlaps it).
 template <int BT, int WM>
 __global__ __launch_bounds__(kThreads3) void k_bins3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
@@ -465,42 +496,87 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
   // one line of margin: the range test and the slice walk round differently
   lfirst = max(lfirst - 1, 0);
   llast = min(llast + 1, L - 1);
+#if DD3D_FWD_PREFETCH
+  // the next line's geometry is loaded while this line is worked on
+  LineGeo3 lg_next = lv[min(lfirst, L - 1)];
+#endif
   for (int l = lfirst; l <= llast; ++l) {
+#if DD3D_FWD_PREFETCH
+    const LineGeo3 lg = lg_next;
+    lg_next = lv[min(l + 1, llast)];
+#else
     const LineGeo3 lg = lv[l];
+#endif
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
     const int cs = max(p1.c, 0), cl = min(p2.c, N - 1);
     if (cs > cl) continue;
     const ZAffine z = z_affine(vg, lg, m);
-    // the chunk's row edges t_r0 .. t_r0+nr in FP32 relative to floor(t_r0)
+    // the chunk's row edges relative to floor(t_r0); rows past nr are empty
     const RowEdges re(z, r0, g.Dz);
-    EdgePos qa = re.pos(0);
-    const EdgePos qend = re.pos(nr);
-    if (qend.c < 0 || qa.c >= g.Dz) continue;  // the chunk's rows miss the slab
+    float t[S + 1];
+#pragma unroll
+    for (int j = 0; j <= S; ++j) t[j] = re.rel(min(j, nr));
+    const int s_lo = max(re.base + static_cast<int>(floorf(t[0])), 0);
+    const int s_hi = min(re.base + static_cast<int>(floorf(t[S])), g.Dz - 1);
+    if (s_lo > s_hi) continue;  // the chunk's rows miss the slab
     const float wl = (WM == W3_FP) ? lg.p * lg.pzf : vg.kbp;
     const float* __restrict__ line = src + static_cast<size_t>(l) * g.Dz * N * g.Bp;
+    // Columns in pairs: each slice's pair is loaded once and its x-weighted
+    // sum spread over the chunk's rows by their z overlaps.  The second
+    // column of a pair past cl is masked (its load stays inside the
+    // buffer, which carries one column of padding).
+    for (int c0 = cs; c0 <= cl; c0 += 2) {
+      const bool two = c0 + 1 <= cl;
+      float w0 = overlap_in_pixel(p1, p2, c0) * wl;
+      float w1 = two ? overlap_in_pixel(p1, p2, c0 + 1) * wl : 0.0f;
+      if constexpr (WM == W3_BPW) {
+        float xc, yc;
+        pixel_centre3(g, xaxis, l, c0, &xc, &yc);
+        w0 *= fbp_weight3(g, vg, xc, yc);
+        pixel_centre3(g, xaxis, l, c0 + 1, &xc, &yc);
+        w1 *= fbp_weight3(g, vg, xc, yc);
+      }
+      const float* __restrict__ px = line + (static_cast<size_t>(s_lo) * N + c0) * g.Bp;
+      const size_t sstride = static_cast<size_t>(N) * g.Bp;
+#if DD3D_FWD_PIPE
+      float x0[BT], x1[BT];
+      load_pair<BT>(px, x0, x1);
+      for (int sl = s_lo; sl <= s_hi; ++sl) {
+        // next slice's pair in flight (the last iteration reloads this one)
+        const float* __restrict__ pn = sl < s_hi ? px + sstride : px;
+        float n0[BT], n1[BT], sx[BT];
+        load_pair<BT>(pn, n0, n1);
+#else
+      for (int sl = s_lo; sl <= s_hi; ++sl, px += sstride) {
+        float x0[BT], x1[BT], sx[BT];
+        load_pair<BT>(px, x0, x1);
+#endif
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      if (j < nr) {
-        const EdgePos qb = (j + 1 == nr) ? qend : re.pos(j + 1);
-        const int ss = max(qa.c, 0), se = min(qb.c, g.Dz - 1);
-        for (int sl = ss; sl <= se; ++sl) {
-          const float wz = overlap_in_pixel(qa, qb, sl) * wl;
-          const float* __restrict__ slab = line + static_cast<size_t>(sl) * N * g.Bp;
-          for (int c = cs; c <= cl; ++c) {
-            float w = overlap_in_pixel(p1, p2, c) * wz;
-            if constexpr (WM == W3_BPW) {
-              float xc, yc;
-              pixel_centre3(g, xaxis, l, c, &xc, &yc);
-              w *= fbp_weight3(g, vg, xc, yc);
-            }
-            float xv[BT];
-            load_tile<BT>(slab + static_cast<size_t>(c) * g.Bp, xv);
-#pragma unroll
-            for (int b = 0; b < BT; ++b) acc[j][b] = fmaf(w, xv[b], acc[j][b]);
-          }
+        for (int b = 0; b < BT; ++b) {
+          sx[b] = w0 * x0[b];
+          sx[b] = two ? fmaf(w1, x1[b], sx[b]) : sx[b];
         }
-        qa = qb;
+        // row j covers [t_j, t_j+1]; its overlap with slice [rs, rs + 1] is
+        // the difference of the two edges clamped to the slice
+        const float rs = static_cast<float>(sl - re.base), rs1 = rs + 1.0f;
+        float clo = __builtin_amdgcn_fmed3f(t[0], rs, rs1);
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const float chi = __builtin_amdgcn_fmed3f(t[j + 1], rs, rs1);
+          const float wz = chi - clo;
+          clo = chi;
+#pragma unroll
+          for (int b = 0; b < BT; ++b) acc[j][b] = fmaf(wz, sx[b], acc[j][b]);
+        }
+#if DD3D_FWD_PIPE
+        px = pn;
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+          x0[b] = n0[b];
+          x1[b] = n1[b];
+        }
+#endif
       }
     }
   }
@@ -709,8 +785,9 @@ Layout3 make_layout3(const Geo3& g, int op) {
   off = 0;
   const size_t vox = static_cast<size_t>(g.W) * g.H * g.Dz * g.Bp;
   if (to_sino3(op)) {
-    L.buf0 = take(sizeof(float) * vox);
-    L.buf1 = take(sizeof(float) * vox);
+    // one column of padding: k_bins3d loads columns in pairs
+    L.buf0 = take(sizeof(float) * (vox + g.Bp));
+    L.buf1 = take(sizeof(float) * (vox + g.Bp));
   } else {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * (g.R + 1) * g.C * g.Bp);
     L.buf1 = L.buf0;
