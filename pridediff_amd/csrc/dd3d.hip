@@ -430,12 +430,6 @@ __device__ __forceinline__ void load_pair(const float* __restrict__ p, float (&x
 #ifndef DD3D_FWD_ROWS
 #define DD3D_FWD_ROWS 4
 #endif
-#ifndef DD3D_FWD_PREFETCH
-#define DD3D_FWD_PREFETCH 0
-#endif
-#ifndef DD3D_FWD_PIPE
-#define DD3D_FWD_PIPE 0
-#endif
 #ifndef DD3D_ADJ_SLICES
 #define DD3D_ADJ_SLICES 16
 #endif
@@ -496,17 +490,11 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
   // one line of margin: the range test and the slice walk round differently
   lfirst = max(lfirst - 1, 0);
   llast = min(llast + 1, L - 1);
-#if DD3D_FWD_PREFETCH
   // the next line's geometry is loaded while this line is worked on
   LineGeo3 lg_next = lv[min(lfirst, L - 1)];
-#endif
   for (int l = lfirst; l <= llast; ++l) {
-#if DD3D_FWD_PREFETCH
     const LineGeo3 lg = lg_next;
     lg_next = lv[min(l + 1, llast)];
-#else
-    const LineGeo3 lg = lv[l];
-#endif
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
     const int cs = max(p1.c, 0), cl = min(p2.c, N - 1);
@@ -539,19 +527,9 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
       }
       const float* __restrict__ px = line + (static_cast<size_t>(s_lo) * N + c0) * g.Bp;
       const size_t sstride = static_cast<size_t>(N) * g.Bp;
-#if DD3D_FWD_PIPE
-      float x0[BT], x1[BT];
-      load_pair<BT>(px, x0, x1);
-      for (int sl = s_lo; sl <= s_hi; ++sl) {
-        // next slice's pair in flight (the last iteration reloads this one)
-        const float* __restrict__ pn = sl < s_hi ? px + sstride : px;
-        float n0[BT], n1[BT], sx[BT];
-        load_pair<BT>(pn, n0, n1);
-#else
       for (int sl = s_lo; sl <= s_hi; ++sl, px += sstride) {
         float x0[BT], x1[BT], sx[BT];
         load_pair<BT>(px, x0, x1);
-#endif
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
           sx[b] = w0 * x0[b];
@@ -569,14 +547,6 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
 #pragma unroll
           for (int b = 0; b < BT; ++b) acc[j][b] = fmaf(wz, sx[b], acc[j][b]);
         }
-#if DD3D_FWD_PIPE
-        px = pn;
-#pragma unroll
-        for (int b = 0; b < BT; ++b) {
-          x0[b] = n0[b];
-          x1[b] = n1[b];
-        }
-#endif
       }
     }
   }
