@@ -220,14 +220,20 @@ __device__ __forceinline__ void bins_accumulate(
     const float* __restrict__ pv, const float* __restrict__ src, float kbp, int l0, int l1,
     int N, float (&acc)[BT]) {
   const int xaxis = vg.xaxis;
+  if (l0 >= l1) return;
+  // the next line's geometry is loaded while this line is worked on
+  LineGeo lg_next = lv[l0];
+  float lw_next = (WM == W_FP) ? pv[l0] : kbp;
   for (int l = l0; l < l1; ++l) {
-    const LineGeo lg = lv[l];
+    const LineGeo lg = lg_next;
+    const float lw = lw_next;
+    lg_next = lv[min(l + 1, l1 - 1)];
+    if constexpr (WM == W_FP) lw_next = pv[min(l + 1, l1 - 1)];
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
     const int cs = max(p1.c, 0);
     const int ce = min(p2.c, N - 1);
     if (cs > ce) continue;
-    const float lw = (WM == W_FP) ? pv[l] : kbp;
     const float* __restrict__ line = src + static_cast<size_t>(l) * N * g.Bp;
     for (int c = cs; c <= ce; ++c) {
       float w = overlap_in_pixel(p1, p2, c) * lw;

@@ -403,29 +403,6 @@ struct RowEdges {
   }
 };
 
-// Two neighbouring voxels of BT instances each (2 BT contiguous floats),
-// one vector load where the width allows.  p is only 4-byte aligned.
-typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
-typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-template <int BT>
-__device__ __forceinline__ void load_pair(const float* __restrict__ p, float (&x0)[BT],
-                                          float (&x1)[BT]) {
-  if constexpr (BT == 1) {
-    const f2u q = *reinterpret_cast<const f2u*>(p);
-    x0[0] = q.x;
-    x1[0] = q.y;
-  } else if constexpr (BT == 2) {
-    const f4u q = *reinterpret_cast<const f4u*>(p);
-    x0[0] = q.x;
-    x0[1] = q.y;
-    x1[0] = q.z;
-    x1[1] = q.w;
-  } else {
-    load_tile<BT>(p, x0);
-    load_tile<BT>(p + BT, x1);
-  }
-}
-
 // Rows (forward) and slices (adjoint) one thread carries in registers.
 #ifndef DD3D_FWD_ROWS
 #define DD3D_FWD_ROWS 4
@@ -529,7 +506,7 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
       const size_t sstride = static_cast<size_t>(N) * g.Bp;
       for (int sl = s_lo; sl <= s_hi; ++sl, px += sstride) {
         float x0[BT], x1[BT], sx[BT];
-        load_pair<BT>(px, x0, x1);
+        load_pair<BT>(px, g.Bp, x0, x1);
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
           sx[b] = w0 * x0[b];

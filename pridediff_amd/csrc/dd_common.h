@@ -100,6 +100,37 @@ __device__ __forceinline__ void load_tile(const float* __restrict__ p,
   }
 }
 
+// Two neighbouring pixels (or voxels) of an interleaved layout whose pixel
+// stride is `stride` floats: BT instances at p and BT at p + stride.  When the
+// two tiles are adjacent (stride == BT, always so for BT <= 2, where the
+// padded batch equals the tile) they come in one vector load; p is then only
+// 4-byte aligned.
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+template <int BT>
+__device__ __forceinline__ void load_pair(const float* __restrict__ p, int stride,
+                                          float (&x0)[BT], float (&x1)[BT]) {
+  if constexpr (BT == 1) {
+    if (stride == 1) {
+      const f2u q = *reinterpret_cast<const f2u*>(p);
+      x0[0] = q.x;
+      x1[0] = q.y;
+      return;
+    }
+  } else if constexpr (BT == 2) {
+    if (stride == 2) {
+      const f4u q = *reinterpret_cast<const f4u*>(p);
+      x0[0] = q.x;
+      x0[1] = q.y;
+      x1[0] = q.z;
+      x1[1] = q.w;
+      return;
+    }
+  }
+  load_tile<BT>(p, x0);
+  load_tile<BT>(p + stride, x1);
+}
+
 template <int BT>
 __device__ __forceinline__ void store_tile(float* __restrict__ p,
                                            const float (&v)[BT]) {
