@@ -12,8 +12,8 @@
 // Launches per call (stream-ordered, no host sync):
 //   k_setup3d            per view: FP64 column edges, magnifications, row-z
 //                        law, per-line x and z maps;
-//   k_interleave_volume  volume -> [line][slice][pixel][b] copies (rows and
-//                        bottom-up columns) for image -> sinogram ops;
+//   k_interleave_volume  volume -> [tile][line][slice][pixel][b] copies (rows
+//                        and bottom-up columns) for image -> sinogram ops;
 //   k_bins3d             image -> sinogram: one thread per detector column and
 //                        chunk of rows, gathering over the lines its ray band
 //                        meets in z;
@@ -259,7 +259,9 @@ __global__ __launch_bounds__(kThreads3) void k_setup3d(
 // ---------------------------------------------------------------------------
 // layout prologues
 // ---------------------------------------------------------------------------
-// x [B][Dz][H][W] -> xrow [H][Dz][W][Bp], xcol [W][Dz][H][Bp] (bottom-up).
+// x [B][Dz][H][W] -> xrow [tile][H][Dz][W][BT], xcol [tile][W][Dz][H][BT]
+// (bottom-up), tile = BT consecutive instances: a pixel's BT values and its
+// neighbour's are contiguous, whatever the padded batch.
 template <int BT>
 __global__ __launch_bounds__(kThreads3) void k_interleave_volume(
     Geo3 g, const float* __restrict__ x, float* __restrict__ xrow,
@@ -276,8 +278,10 @@ __global__ __launch_bounds__(kThreads3) void k_interleave_volume(
     v[b] = (b0 + b < g.B)
                ? x[((static_cast<size_t>(b0 + b) * g.Dz + s) * g.H + h) * g.W + w]
                : 0.0f;
-  store_tile<BT>(xrow + ((static_cast<size_t>(h) * g.Dz + s) * g.W + w) * g.Bp + b0, v);
-  store_tile<BT>(xcol + ((static_cast<size_t>(w) * g.Dz + s) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
+  const size_t tile = static_cast<size_t>(blockIdx.z) * g.W * g.H * g.Dz * BT;
+  store_tile<BT>(xrow + tile + ((static_cast<size_t>(h) * g.Dz + s) * g.W + w) * BT, v);
+  store_tile<BT>(xcol + tile + ((static_cast<size_t>(w) * g.Dz + s) * g.H + (g.H - 1 - h)) * BT,
+                 v);
 }
 
 // p [B][V][R][C] -> Q [V][R+1][C][Bp]: per view and column, the running sum
@@ -452,7 +456,8 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
   const int xaxis = vg.xaxis;
   const int L = xaxis ? g.H : g.W;
   const int N = xaxis ? g.W : g.H;
-  const float* __restrict__ src = (xaxis ? xrow : xcol) + b0;
+  const float* __restrict__ src =
+      (xaxis ? xrow : xcol) + static_cast<size_t>(b0) * g.W * g.H * g.Dz;  // tile b0 / BT
   const double* ce = cedge + static_cast<size_t>(v) * (g.C + 1);
   const double e1 = ce[k], e2 = ce[k + 1];
   const double m = mag[static_cast<size_t>(v) * g.C + k];
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
     const int s_hi = min(re.base + static_cast<int>(floorf(t[S])), g.Dz - 1);
     if (s_lo > s_hi) continue;  // the chunk's rows miss the slab
     const float wl = (WM == W3_FP) ? lg.p * lg.pzf : vg.kbp;
-    const float* __restrict__ line = src + static_cast<size_t>(l) * g.Dz * N * g.Bp;
+    const float* __restrict__ line = src + static_cast<size_t>(l) * g.Dz * N * BT;
     // Columns in pairs: each slice's pair is loaded once and its x-weighted
     // sum spread over the chunk's rows by their z overlaps.  The second
     // column of a pair past cl is masked (its load stays inside the
@@ -502,11 +507,11 @@ __global__ __launch_bounds__(kThreads3) void k_bins3d(
         pixel_centre3(g, xaxis, l, c0 + 1, &xc, &yc);
         w1 *= fbp_weight3(g, vg, xc, yc);
       }
-      const float* __restrict__ px = line + (static_cast<size_t>(s_lo) * N + c0) * g.Bp;
-      const size_t sstride = static_cast<size_t>(N) * g.Bp;
+      const float* __restrict__ px = line + (static_cast<size_t>(s_lo) * N + c0) * BT;
+      const size_t sstride = static_cast<size_t>(N) * BT;
       for (int sl = s_lo; sl <= s_hi; ++sl, px += sstride) {
         float x0[BT], x1[BT], sx[BT];
-        load_pair<BT>(px, g.Bp, x0, x1);
+        load_pair<BT>(px, BT, x0, x1);
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
           sx[b] = w0 * x0[b];
@@ -732,9 +737,9 @@ Layout3 make_layout3(const Geo3& g, int op) {
   off = 0;
   const size_t vox = static_cast<size_t>(g.W) * g.H * g.Dz * g.Bp;
   if (to_sino3(op)) {
-    // one column of padding: k_bins3d loads columns in pairs
-    L.buf0 = take(sizeof(float) * (vox + g.Bp));
-    L.buf1 = take(sizeof(float) * (vox + g.Bp));
+    // one pixel of padding: k_bins3d loads pixels in pairs
+    L.buf0 = take(sizeof(float) * (vox + g.BT));
+    L.buf1 = take(sizeof(float) * (vox + g.BT));
   } else {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * (g.R + 1) * g.C * g.Bp);
     L.buf1 = L.buf0;
