@@ -284,10 +284,11 @@ __global__ __launch_bounds__(kThreads3) void k_interleave_volume(
                  v);
 }
 
-// p [B][V][R][C] -> Q [V][R+1][C][Bp]: per view and column, the running sum
+// p [B][V][R][C] -> Q [V][R+2][C][Bp]: per view and column, the running sum
 // over detector rows (columns in walk order, optionally times the FP weight
-// of each cell), Q[0] = 0.  Accumulated in FP64 and stored rounded, so every
-// entry carries one rounding and no drift.
+// of each cell), Q[0] = 0, and Q[R+1] = Q[R] so an interpolation at the top
+// edge (row R, fraction 0) needs no special case.  Accumulated in FP64 and
+// stored rounded, so every entry carries one rounding and no drift.
 template <int BT, bool FOLD_W>
 __global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
     m = mag[static_cast<size_t>(v) * g.C + k];
   }
   const size_t rstride = static_cast<size_t>(g.C) * g.Bp;
-  float* __restrict__ qv = q + static_cast<size_t>(v) * (g.R + 1) * rstride +
+  float* __restrict__ qv = q + static_cast<size_t>(v) * (g.R + 2) * rstride +
                            static_cast<size_t>(k) * g.Bp + b0;
   double acc[BT];
   float val[BT];
@@ -336,6 +337,7 @@ __global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
     }
     store_tile<BT>(qv + (r + 1) * rstride, val);
   }
+  store_tile<BT>(qv + (g.R + 1) * rstride, val);
 }
 
 // ---------------------------------------------------------------------------
@@ -653,20 +655,23 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
       const float ulo = static_cast<float>(-rbase), uhi = static_cast<float>(g.R - rbase);
       if (uhi > 0.0f && fmaf(static_cast<float>(S), invb, du) > ulo) {
         const float wxb = wx * static_cast<float>(z.B);
-        const float* __restrict__ qk = sino + (static_cast<size_t>(v) * (g.R + 1) * g.C + k) * g.Bp + b0;
-        const size_t rstride = static_cast<size_t>(g.C) * g.Bp;
+        // the view's block of Q (wave-uniform base) plus 32-bit byte
+        // offsets: a block holds (R + 2) C Bp < 2^30 floats (checked on the
+        // host), so base + offset maps onto scalar-base vector loads
+        const char* __restrict__ qv = reinterpret_cast<const char*>(
+            sino + static_cast<size_t>(v) * (g.R + 2) * g.C * g.Bp);
+        const unsigned rstride = 4u * static_cast<unsigned>(g.C * g.Bp);
+        const unsigned okb =
+            4u * static_cast<unsigned>(k * g.Bp + b0) + static_cast<unsigned>(rbase) * rstride;
         auto eval = [&](int j, float (&l)[BT]) {
-          const float u = fminf(fmaxf(fmaf(static_cast<float>(j), invb, du), ulo), uhi);
+          // u in [-rbase, R - rbase]: rows r, r + 1 with r <= R (Q[R+1] = Q[R])
+          const float u = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(j), invb, du), ulo, uhi);
           const float fu = floorf(u);
-          int r = rbase + static_cast<int>(fu);
-          float fr = u - fu;
-          if (r >= g.R) {  // at the top edge: the last row's end
-            r = g.R - 1;
-            fr = 1.0f;
-          }
+          const float fr = u - fu;
+          const unsigned off = okb + static_cast<unsigned>(static_cast<int>(fu)) * rstride;
           float q0[BT], q1[BT];
-          load_tile<BT>(qk + r * rstride, q0);
-          load_tile<BT>(qk + (r + 1) * rstride, q1);
+          load_tile<BT>(reinterpret_cast<const float*>(qv + off), q0);
+          load_tile<BT>(reinterpret_cast<const float*>(qv + (off + rstride)), q1);
 #pragma unroll
           for (int b = 0; b < BT; ++b) l[b] = fmaf(fr, q1[b] - q0[b], q0[b]);
         };
@@ -741,7 +746,7 @@ Layout3 make_layout3(const Geo3& g, int op) {
     L.buf0 = take(sizeof(float) * (vox + g.BT));
     L.buf1 = take(sizeof(float) * (vox + g.BT));
   } else {
-    L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * (g.R + 1) * g.C * g.Bp);
+    L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * (g.R + 2) * g.C * g.Bp);
     L.buf1 = L.buf0;
   }
   L.data_total = off;
@@ -777,6 +782,8 @@ bool fill_geo3(Geo3* out, int scan, const dd_geom3d* g, int batch, int views) {
   q.shz = g->shift_z;
   q.csh = g->col_shift;
   q.rsh = g->row_shift;
+  // the adjoint addresses one view's row running sums with 32-bit byte offsets
+  if (static_cast<long long>(q.R + 2) * q.C * q.Bp >= (1LL << 30)) return false;
   *out = q;
   return true;
 }
