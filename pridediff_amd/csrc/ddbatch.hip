@@ -34,7 +34,23 @@ namespace {
 
 using namespace body2d;
 
-// Device descriptor of one instance.
+// Device descriptor of one instance, as stored in the descriptor table at the
+// start of the batch's device block: its tables and scratch are byte offsets
+// into that block.  Kernels take the block's base as an argument and rebuild
+// the pointers from it (BatchInst), so every access is a global-memory one
+// (pointers read from memory would otherwise be generic: flat loads).
+enum Region { kViews, kEdges, kWfp, kLines, kPint, kXrow, kXcol, kSino, kPart, kRegions };
+
+struct BatchDesc {
+  Geo2 g;
+  unsigned long long off[kRegions];
+  long long img_off;   // floats, into the packed image buffer
+  long long sino_off;  // floats, into the packed sinogram buffer
+  int item0[6];        // first work item of each phase (see Phase)
+  int nitems[6];
+};
+
+// One instance inside a kernel: the descriptor with typed pointers.
 struct BatchInst {
   Geo2 g;
   ViewGeo* views;
@@ -46,10 +62,8 @@ struct BatchInst {
   float* xcol;         // layout scratch: image columns, bottom-up
   float* sino;         // layout scratch: sinogram in walk order
   float* part;         // partial bin sums per line chunk [kSplit][B][V][D]
-  long long img_off;   // floats, into the packed image buffer
-  long long sino_off;  // floats, into the packed sinogram buffer
-  int item0[6];        // first work item of each phase (see Phase)
-  int nitems[6];
+  long long img_off;
+  long long sino_off;
 };
 
 enum Phase {
@@ -64,28 +78,53 @@ enum Phase {
 
 constexpr int kBT = 1;  // one geometry per instance
 
-__device__ __forceinline__ int find_inst(const BatchInst* __restrict__ inst, int n,
-                                         int phase, int item) {
+__device__ __forceinline__ int find_inst(const char* __restrict__ mem, int n, int phase,
+                                         int item) {
+  const BatchDesc* __restrict__ desc = reinterpret_cast<const BatchDesc*>(mem);
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) / 2;
-    if (inst[mid].item0[phase] <= item) lo = mid; else hi = mid - 1;
+    if (desc[mid].item0[phase] <= item) lo = mid; else hi = mid - 1;
   }
   return lo;
 }
 
+__device__ __forceinline__ BatchInst load_inst(char* __restrict__ mem, int i) {
+  const BatchDesc& s = reinterpret_cast<const BatchDesc*>(mem)[i];
+  BatchInst d;
+  d.g = s.g;
+  d.views = reinterpret_cast<ViewGeo*>(mem + s.off[kViews]);
+  d.edges = reinterpret_cast<double*>(mem + s.off[kEdges]);
+  d.wfp = reinterpret_cast<float*>(mem + s.off[kWfp]);
+  d.lines = reinterpret_cast<LineGeo*>(mem + s.off[kLines]);
+  d.pint = reinterpret_cast<float*>(mem + s.off[kPint]);
+  d.xrow = reinterpret_cast<float*>(mem + s.off[kXrow]);
+  d.xcol = reinterpret_cast<float*>(mem + s.off[kXcol]);
+  d.sino = reinterpret_cast<float*>(mem + s.off[kSino]);
+  d.part = reinterpret_cast<float*>(mem + s.off[kPart]);
+  d.img_off = s.img_off;
+  d.sino_off = s.sino_off;
+  return d;
+}
+
+// First work item of instance i in a phase (read from the table, not the
+// copy, so a runtime phase index does not put the copy in scratch memory).
+__device__ __forceinline__ int item0_of(const char* __restrict__ mem, int i, int phase) {
+  return reinterpret_cast<const BatchDesc*>(mem)[i].item0[phase];
+}
+
 __global__ __launch_bounds__(kThreads) void k_batch_setup(
-    const BatchInst* __restrict__ inst, const float* const* __restrict__ angles,
-    int n_inst, int total) {
+    char* __restrict__ mem, const float* const* __restrict__ angles, int n_inst, int total) {
+  const BatchDesc* __restrict__ desc = reinterpret_cast<const BatchDesc*>(mem);
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     // setup items: one per (instance, view); item0[kLayoutImage] is unused here,
     // so the per-instance view offsets are recomputed by a linear scan
     int i = 0, base = 0;
-    while (i < n_inst - 1 && item >= base + inst[i].g.V) {
-      base += inst[i].g.V;
+    while (i < n_inst - 1 && item >= base + desc[i].g.V) {
+      base += desc[i].g.V;
       ++i;
     }
-    const BatchInst& d = inst[i];
+    const BatchInst d = load_inst(mem, i);
     setup2d_view(d.g, angles[i], d.views, d.edges, d.wfp, d.lines, d.pint, item - base);
     __syncthreads();
   }
@@ -93,12 +132,12 @@ __global__ __launch_bounds__(kThreads) void k_batch_setup(
 
 template <bool FOLD_W>
 __global__ __launch_bounds__(kThreads) void k_batch_layout(
-    const BatchInst* __restrict__ inst, int n_inst, int phase, int total,
+    char* __restrict__ mem, int n_inst, int phase, int total,
     const float* __restrict__ in) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int i = find_inst(inst, n_inst, phase, item);
-    const BatchInst& d = inst[i];
-    const int local = item - d.item0[phase];
+    const int i = find_inst(mem, n_inst, phase, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, phase);
     if (phase == kLayoutImage) {
       // item = (image row, instance tile); threads stride over the row
       const int h = local % d.g.H;
@@ -124,11 +163,11 @@ constexpr int kBinsWave = 64;
 
 template <int WM>
 __global__ __launch_bounds__(kBinsWave) void k_batch_bins(
-    const BatchInst* __restrict__ inst, int n_inst, int total) {
+    char* __restrict__ mem, int n_inst, int total) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int i = find_inst(inst, n_inst, kGatherBins, item);
-    const BatchInst& d = inst[i];
-    const int local = item - d.item0[kGatherBins];
+    const int i = find_inst(mem, n_inst, kGatherBins, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kGatherBins);
     const int chunk = local % kSplit;
     int rest = local / kSplit;
     const int kb_n = (d.g.D + kBinsWave - 1) / kBinsWave;
@@ -159,11 +198,11 @@ __global__ __launch_bounds__(kBinsWave) void k_batch_bins(
 // the bin weight, detector order (the arithmetic of k_bins_reduce).
 template <int WM>
 __global__ __launch_bounds__(kThreads) void k_batch_reduce(
-    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out) {
+    char* __restrict__ mem, int n_inst, int total, float* __restrict__ out) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int i = find_inst(inst, n_inst, kReduceBins, item);
-    const BatchInst& d = inst[i];
-    const int local = item - d.item0[kReduceBins];
+    const int i = find_inst(mem, n_inst, kReduceBins, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kReduceBins);
     const int kb_n = (d.g.D + kThreads - 1) / kThreads;
     const int k = (local % kb_n) * kThreads + threadIdx.x;
     const int v = (local / kb_n) % d.g.V;
@@ -183,13 +222,13 @@ __global__ __launch_bounds__(kThreads) void k_batch_reduce(
 
 template <int WM>
 __global__ __launch_bounds__(kThreads) void k_batch_pixels(
-    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out,
+    char* __restrict__ mem, int n_inst, int total, float* __restrict__ out,
     int per_pixel) {
   __shared__ PixelTileLds<kTile> lds;
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int i = find_inst(inst, n_inst, kGatherPixels, item);
-    const BatchInst& d = inst[i];
-    const int local = item - d.item0[kGatherPixels];
+    const int i = find_inst(mem, n_inst, kGatherPixels, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kGatherPixels);
     const int tx = (d.g.W + kTile - 1) / kTile, ty = (d.g.H + kTile - 1) / kTile;
     const int t = local % (tx * ty);
     const int b0 = (local / (tx * ty)) * kBT;
@@ -209,11 +248,11 @@ __global__ __launch_bounds__(kThreads) void k_batch_pixels(
 // Quad items: (instance, 32 x 32 pixel tile), one thread per 2 x 2 pixels.
 template <int WM>
 __global__ __launch_bounds__(kThreads) void k_batch_quads(
-    const BatchInst* __restrict__ inst, int n_inst, int total, float* __restrict__ out) {
+    char* __restrict__ mem, int n_inst, int total, float* __restrict__ out) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int i = find_inst(inst, n_inst, kGatherQuads, item);
-    const BatchInst& d = inst[i];
-    const int local = item - d.item0[kGatherQuads];
+    const int i = find_inst(mem, n_inst, kGatherQuads, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kGatherQuads);
     const int tx = (d.g.W + 2 * kTile - 1) / (2 * kTile);
     const int ty = (d.g.H + 2 * kTile - 1) / (2 * kTile);
     const int t = local % (tx * ty);
@@ -233,7 +272,7 @@ using namespace dd;
 
 struct dd2d_batch {
   std::vector<dd2d_batch_item> items;
-  std::vector<BatchInst> inst;   // host copy of the descriptors
+  std::vector<BatchDesc> inst;   // host copy of the descriptors
   std::vector<int> order;        // instance order (heaviest first)
   size_t desc_off = 0, angles_off = 0, bytes = 0;
   size_t total_views = 0;
@@ -285,13 +324,13 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
            1.0 * c.views * c.geom.width * c.geom.height * c.batch;
   });
   size_t off = 0;
-  b->desc_off = take(&off, sizeof(BatchInst) * n);
+  b->desc_off = take(&off, sizeof(BatchDesc) * n);  // 0: kernels find it at the block base
   b->angles_off = take(&off, sizeof(float*) * n);
   b->inst.resize(n);
   int acc[kPhases] = {0, 0, 0, 0, 0, 0};
   for (int j = 0; j < n; ++j) {
     const auto& it = items[b->order[j]];
-    BatchInst d{};
+    BatchDesc d{};
     Geo2& g = d.g;
     g.W = it.geom.width;
     g.H = it.geom.height;
@@ -310,18 +349,18 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     g.shx = it.geom.shift_x;
     g.shy = it.geom.shift_y;
     g.bin = it.geom.bin_shift;
-    // geometry tables and layout scratch (offsets now, pointers at setup)
-    d.views = reinterpret_cast<ViewGeo*>(take(&off, sizeof(ViewGeo) * g.V));
-    d.edges = reinterpret_cast<double*>(take(&off, sizeof(double) * g.V * (g.D + 1)));
-    d.wfp = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.D));
-    d.lines = reinterpret_cast<LineGeo*>(take(&off, sizeof(LineGeo) * g.V * g.Lmax));
-    d.pint = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.Lmax));
+    // geometry tables and layout scratch: byte offsets into the device block
+    d.off[kViews] = take(&off, sizeof(ViewGeo) * g.V);
+    d.off[kEdges] = take(&off, sizeof(double) * g.V * (g.D + 1));
+    d.off[kWfp] = take(&off, sizeof(float) * g.V * g.D);
+    d.off[kLines] = take(&off, sizeof(LineGeo) * g.V * g.Lmax);
+    d.off[kPint] = take(&off, sizeof(float) * g.V * g.Lmax);
     const size_t img = static_cast<size_t>(g.W) * g.H * g.Bp;
-    d.xrow = reinterpret_cast<float*>(take(&off, sizeof(float) * img));
-    d.xcol = reinterpret_cast<float*>(take(&off, sizeof(float) * img));
-    d.sino = reinterpret_cast<float*>(take(&off, sizeof(float) * g.V * g.D * g.Bp));
-    d.part = reinterpret_cast<float*>(
-        take(&off, sizeof(float) * static_cast<size_t>(kSplit) * g.Bp * g.V * g.D));
+    d.off[kXrow] = take(&off, sizeof(float) * img);
+    d.off[kXcol] = take(&off, sizeof(float) * img);
+    d.off[kSino] = take(&off, sizeof(float) * g.V * g.D * g.Bp);
+    d.off[kPart] =
+        take(&off, sizeof(float) * static_cast<size_t>(kSplit) * g.Bp * g.V * g.D);
     d.img_off = it.image_offset;
     d.sino_off = it.sino_offset;
     const int tiles = it.batch;  // kBT == 1
@@ -365,33 +404,15 @@ dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* 
   b->dev = static_cast<char*>(device_mem);
   auto st = static_cast<hipStream_t>(stream);
   const int n = static_cast<int>(b->inst.size());
-  std::vector<BatchInst> host(b->inst);
   std::vector<const float*> angles(n);
-  auto rebase = [&](auto* p) {
-    return reinterpret_cast<decltype(p)>(b->dev + reinterpret_cast<size_t>(p));
-  };
-  for (int j = 0; j < n; ++j) {
-    BatchInst& d = host[j];
-    d.views = rebase(d.views);
-    d.edges = rebase(d.edges);
-    d.wfp = rebase(d.wfp);
-    d.lines = rebase(d.lines);
-    d.pint = rebase(d.pint);
-    d.xrow = rebase(d.xrow);
-    d.xcol = rebase(d.xcol);
-    d.sino = rebase(d.sino);
-    d.part = rebase(d.part);
-    angles[j] = b->items[b->order[j]].angles;
-  }
-  b->inst = host;
-  if (hipMemcpyAsync(b->dev + b->desc_off, host.data(), sizeof(BatchInst) * n,
+  for (int j = 0; j < n; ++j) angles[j] = b->items[b->order[j]].angles;
+  if (hipMemcpyAsync(b->dev + b->desc_off, b->inst.data(), sizeof(BatchDesc) * n,
                      hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(b->dev + b->angles_off, angles.data(), sizeof(float*) * n,
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return DD_LAUNCH_FAIL;
   const int total = static_cast<int>(b->total_views);
-  hipLaunchKernelGGL(k_batch_setup, dim3(grid_for(b, total)), dim3(kThreads), 0, st,
-                     reinterpret_cast<const BatchInst*>(b->dev + b->desc_off),
+  hipLaunchKernelGGL(k_batch_setup, dim3(grid_for(b, total)), dim3(kThreads), 0, st, b->dev,
                      reinterpret_cast<const float* const*>(b->dev + b->angles_off), n, total);
   if (hipGetLastError() != hipSuccess) return DD_LAUNCH_FAIL;
   // the host staging vectors die with this call: wait for the uploads
@@ -406,7 +427,7 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
     if (it.scan == DD_PARA && op >= DD_BPROJ_W) return DD_UNSUPPORTED;
   auto st = static_cast<hipStream_t>(stream);
   const int n = static_cast<int>(b->inst.size());
-  const auto* inst = reinterpret_cast<const BatchInst*>(b->dev + b->desc_off);
+  char* inst = b->dev;  // descriptor table at the block base
   const bool sino_out = (op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW);
   if (sino_out) {
     hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutImage])),
