@@ -40,6 +40,10 @@ typedef struct {
   float row_shift;
 } dd_geom3d;
 
+/* Workspace bytes for one call (0 for an invalid geometry).  Geometries
+ * whose per-view detector block (rows + 2) * cols * padded batch reaches
+ * 2^30 floats are rejected (0 here, DD_BAD_ARG from dd3d_run): the
+ * sinogram -> volume gather addresses a view's block with 32-bit offsets. */
 size_t dd3d_workspace_size(int scan, int op, const dd_geom3d* g, int batch,
                            int views);
 
