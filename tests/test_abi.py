@@ -67,6 +67,30 @@ def test_workspace_queries_are_host_only():
     assert lib.dd2d_run(9, 0, ctypes.byref(g), None, None, None, 1, 1, None, 0, None) == 1
 
 
+def test_3d_workspace_query_and_size_limit():
+    """3D workspace sizes are host arithmetic; a per-view detector block of
+    2^30 floats or more is rejected (the adjoint addresses it with 32-bit
+    offsets)."""
+    lib = ctypes.CDLL(LIB)
+
+    class G3(ctypes.Structure):
+        _fields_ = [(n, ctypes.c_int) for n in ("width", "height", "depth", "num_col", "num_row")] + \
+                   [(n, ctypes.c_float) for n in ("pix_x", "pix_y", "pix_z", "col_size", "row_size",
+                                                  "iso_source", "source_detector", "shift_x", "shift_y",
+                                                  "shift_z", "col_shift", "row_shift")]
+    lib.dd3d_workspace_size.restype = ctypes.c_size_t
+    g = G3(256, 256, 256, 512, 384, 1.0, 1.0, 1.0, 1.0, 1.0, 1000.0, 1500.0, 0, 0, 0, 0, 0)
+    fp = lib.dd3d_workspace_size(3, 0, ctypes.byref(g), 2, 360)
+    adj = lib.dd3d_workspace_size(3, 1, ctypes.byref(g), 2, 360)
+    # two interleaved volume copies (+ one pixel of padding each) / row running sums
+    assert fp >= 2 * (256 ** 3 * 2 + 2) * 4
+    assert adj >= 360 * (384 + 2) * 512 * 2 * 4
+    big = G3(64, 64, 64, 32768, 32768, 1.0, 1.0, 1.0, 1.0, 1.0, 1000.0, 1500.0, 0, 0, 0, 0, 0)
+    assert lib.dd3d_workspace_size(3, 1, ctypes.byref(big), 1, 4) == 0
+    big.num_row = (1 << 30) // 32768 - 3  # (rows + 2) * cols just below 2^30
+    assert lib.dd3d_workspace_size(3, 1, ctypes.byref(big), 1, 4) > 0
+
+
 def test_radontorch_registers_reference_ops():
     from pridediff_amd import load_radontorch
     rt = load_radontorch()
