@@ -166,6 +166,26 @@ def test_autograd_gradient_and_finite_differences(rt):
     assert abs(fd - an) <= 1e-3 * abs(an)
 
 
+def test_autocast_inputs_are_cast_to_float32(rt):
+    """Under autocast the autograd Functions cast bf16 inputs to FP32
+    (custom_fwd(cast_inputs=float32), function.py:4-9 of the reference): the
+    result is FP32 and equals the FP32 call on the rounded input, and the
+    backward runs."""
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    n, d, v = 64, 92, 60
+    P = projector2d(n, d, 0.7 * 8, 1.2858 * 8, 595.0, 1085.6, scan_type="arc").cuda()
+    ang = torch.from_numpy(_angles(v)).cuda()
+    x = torch.from_numpy(ellipses2d(n, 2, seed=15, scale=0.04))[:, None].cuda()
+    xb = x.to(torch.bfloat16).requires_grad_(True)
+    with torch.autocast("cuda", dtype=torch.bfloat16):
+        out = P.projection(xb, ang)
+    assert out.dtype == torch.float32
+    want = P.projection(xb.detach().float(), ang)
+    assert torch.equal(out, want)
+    out.sum().backward()
+    assert xb.grad is not None and xb.grad.dtype == torch.bfloat16
+
+
 @pytest.mark.parametrize("geom", ["para", "flat", "arc"])
 def test_system_matrix_matches_oracle(rt, geom):
     z, g = _golden(geom)
