@@ -107,27 +107,31 @@ __global__ __launch_bounds__(kThreads) void k_bins_split(
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
     const float* __restrict__ xrow, const float* __restrict__ xcol,
     float* __restrict__ part) {
-  const int kb_n = (g.D + kThreads - 1) / kThreads;
+  constexpr int kBinsPerBlock = (kThreads / 64) * kBinsPerWave;
+  const int kb_n = (g.D + kBinsPerBlock - 1) / kBinsPerBlock;
   const int chunk = blockIdx.x % kSplit;
   int rest = blockIdx.x / kSplit;
   const int kb = rest % kb_n;
   rest /= kb_n;
   const int v = rest % g.V;
   const int b0 = (rest / g.V) * BT;
-  const int k = kb * kThreads + threadIdx.x;
-  if (k >= g.D) return;
+  const int lane = static_cast<int>(threadIdx.x % 64);
+  const int k = kb * kBinsPerBlock + static_cast<int>(threadIdx.x / 64) * kBinsPerWave + lane;
+  const bool edge_lane = lane == 63;
+  // every lane stays to the end (the edge exchange needs the whole wave);
+  // lanes past the last bin work on clamped edges and write nothing
   const ViewGeo vg = views[v];
   const int L = vg.xaxis ? g.H : g.W;
   const int N = vg.xaxis ? g.W : g.H;
   const int l0 = L * chunk / kSplit, l1 = L * (chunk + 1) / kSplit;
-  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + k];
-  const double e2 = edges[static_cast<size_t>(v) * (g.D + 1) + k + 1];
+  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + min(k, g.D)];
   float acc[BT];
 #pragma unroll
   for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
-  bins_accumulate<BT, WM>(g, vg, e1, e2, lines + static_cast<size_t>(v) * g.Lmax,
+  bins_accumulate<BT, WM>(g, vg, e1, lines + static_cast<size_t>(v) * g.Lmax,
                           pint + static_cast<size_t>(v) * g.Lmax,
-                          (vg.xaxis ? xrow : xcol) + b0, bp_const(g), l0, l1, N, acc);
+                          (vg.xaxis ? xrow : xcol) + b0, bp_const(g), l0, l1, N, acc, edge_lane);
+  if (edge_lane || k >= g.D) return;
 #pragma unroll
   for (int b = 0; b < BT; ++b)
     if (b0 + b < g.B)
@@ -390,7 +394,8 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
                        dim3(kThreads), 0, st, g, input, buf0, buf1);
     auto* part = reinterpret_cast<float*>(ws + L.part);
     const int kb_n = (g.D + kThreads - 1) / kThreads;
-    dim3 pgrid(kSplit * kb_n * g.V * ntile);
+    const int kbs_n = (g.D + (kThreads / 64) * kBinsPerWave - 1) / ((kThreads / 64) * kBinsPerWave);
+    dim3 pgrid(kSplit * kbs_n * g.V * ntile);
     dim3 rgrid(kb_n * g.V * g.B);
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ) {

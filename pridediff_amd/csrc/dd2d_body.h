@@ -213,12 +213,32 @@ __device__ __forceinline__ void pixel_centre(const Geo2& g, int xaxis, int l,
 // (one per XCD) and add the chunk sums in chunk order in a second pass.
 constexpr int kSplit = 8;
 
-// Sum over lines [l0, l1) of one bin's overlaps with the image (no bin weight).
+// Forward gathers run 63 bins per wavefront: consecutive lanes hold
+// consecutive bins, every lane evaluates the position of its lower edge on
+// each line and takes its upper edge's from the next lane (a DPP wave shift,
+// the same bits), and lane 63 only serves lane 62's upper edge.  Each edge is
+// evaluated once per line instead of twice.
+constexpr int kBinsPerWave = 63;
+
+__device__ __forceinline__ EdgePos edge_from_next_lane(EdgePos p) {
+  constexpr int kWaveShl1 = 0x130;  // DPP wave_shl:1: lane i reads lane i + 1
+  EdgePos q;
+  q.c = __builtin_amdgcn_update_dpp(0, p.c, kWaveShl1, 0xf, 0xf, false);
+  q.f = __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, p.f), kWaveShl1, 0xf, 0xf,
+                                         false));
+  return q;
+}
+
+// Sum over lines [l0, l1) of one bin's overlaps with the image (no bin
+// weight); e1 is the bin's lower edge.  The whole wavefront calls this
+// together (every lane active: the DPP reads the next lane); `edge_lane`
+// marks lane 63.
 template <int BT, int WM>
 __device__ __forceinline__ void bins_accumulate(
-    const Geo2& g, const ViewGeo& vg, double e1, double e2, const LineGeo* __restrict__ lv,
+    const Geo2& g, const ViewGeo& vg, double e1, const LineGeo* __restrict__ lv,
     const float* __restrict__ pv, const float* __restrict__ src, float kbp, int l0, int l1,
-    int N, float (&acc)[BT]) {
+    int N, float (&acc)[BT], bool edge_lane) {
   const int xaxis = vg.xaxis;
   if (l0 >= l1) return;
   // the next line's geometry is loaded while this line is worked on
@@ -230,7 +250,8 @@ __device__ __forceinline__ void bins_accumulate(
     lg_next = lv[min(l + 1, l1 - 1)];
     if constexpr (WM == W_FP) lw_next = pv[min(l + 1, l1 - 1)];
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
-    const EdgePos p2 = edge_pos(e2, lg.a0, lg.inv_p, N);
+    const EdgePos p2 = edge_from_next_lane(p1);
+    if (edge_lane) continue;
     const int cs = max(p1.c, 0);
     const int ce = min(p2.c, N - 1);
     if (cs > ce) continue;

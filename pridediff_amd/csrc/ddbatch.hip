@@ -154,12 +154,12 @@ __global__ __launch_bounds__(kThreads) void k_batch_layout(
   }
 }
 
-// Gather-bins items: (64-bin block, view, instance, line chunk), chunk
-// fastest, one wavefront per item (a 64-thread workgroup), so instances whose
-// bin count is not a multiple of 256 keep their lanes busy.  Every instance's
-// item count and first item are multiples of kSplit, so with a grid that is a
-// multiple of 8 the chunk follows the XCD.
-constexpr int kBinsWave = 64;
+// Gather-bins items: (block of kBinsPerWave bins, view, instance, line
+// chunk), chunk fastest, one wavefront per item (a 64-thread workgroup), so
+// instances whose bin count is not a multiple of 256 keep their lanes busy.
+// Every instance's item count and first item are multiples of kSplit, so with
+// a grid that is a multiple of 8 the chunk follows the XCD.
+constexpr int kBinsWave = 64;  // threads per gather-bins workgroup
 
 template <int WM>
 __global__ __launch_bounds__(kBinsWave) void k_batch_bins(
@@ -170,24 +170,27 @@ __global__ __launch_bounds__(kBinsWave) void k_batch_bins(
     const int local = item - item0_of(mem, i, kGatherBins);
     const int chunk = local % kSplit;
     int rest = local / kSplit;
-    const int kb_n = (d.g.D + kBinsWave - 1) / kBinsWave;
-    const int k = (rest % kb_n) * kBinsWave + threadIdx.x;
+    const int kb_n = (d.g.D + kBinsPerWave - 1) / kBinsPerWave;
+    const int lane = static_cast<int>(threadIdx.x);
+    const int k = (rest % kb_n) * kBinsPerWave + lane;
+    const bool edge_lane = lane == 63;
     rest /= kb_n;
     const int v = rest % d.g.V;
     const int b0 = (rest / d.g.V) * kBT;
-    if (k >= d.g.D) continue;
+    // the whole wave works every item (the edge exchange needs it); lanes
+    // past the last bin use clamped edges and write nothing
     const ViewGeo vg = d.views[v];
     const int L = vg.xaxis ? d.g.H : d.g.W;
     const int N = vg.xaxis ? d.g.W : d.g.H;
-    const double e1 = d.edges[static_cast<size_t>(v) * (d.g.D + 1) + k];
-    const double e2 = d.edges[static_cast<size_t>(v) * (d.g.D + 1) + k + 1];
+    const double e1 = d.edges[static_cast<size_t>(v) * (d.g.D + 1) + min(k, d.g.D)];
     float acc[kBT];
 #pragma unroll
     for (int b = 0; b < kBT; ++b) acc[b] = 0.0f;
-    bins_accumulate<kBT, WM>(d.g, vg, e1, e2, d.lines + static_cast<size_t>(v) * d.g.Lmax,
+    bins_accumulate<kBT, WM>(d.g, vg, e1, d.lines + static_cast<size_t>(v) * d.g.Lmax,
                              d.pint + static_cast<size_t>(v) * d.g.Lmax,
                              (vg.xaxis ? d.xrow : d.xcol) + b0, bp_const(d.g),
-                             L * chunk / kSplit, L * (chunk + 1) / kSplit, N, acc);
+                             L * chunk / kSplit, L * (chunk + 1) / kSplit, N, acc, edge_lane);
+    if (edge_lane || k >= d.g.D) continue;
 #pragma unroll
     for (int b = 0; b < kBT; ++b)
       d.part[((static_cast<size_t>(chunk) * d.g.Bp + b0 + b) * d.g.V + v) * d.g.D + k] = acc[b];
@@ -365,7 +368,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     d.sino_off = it.sino_offset;
     const int tiles = it.batch;  // kBT == 1
     d.nitems[kLayoutImage] = g.H * tiles;
-    d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kBinsWave - 1) / kBinsWave) * tiles;
+    d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kBinsPerWave - 1) / kBinsPerWave) * tiles;
     d.nitems[kReduceBins] = g.V * ((g.D + kThreads - 1) / kThreads) * it.batch;
     d.nitems[kGatherQuads] =
         ((g.W + 2 * kTile - 1) / (2 * kTile)) * ((g.H + 2 * kTile - 1) / (2 * kTile)) * tiles;
