@@ -111,6 +111,25 @@ def test_uneven_batch_equals_per_instance_ops():
             assert torch.equal(b2, P.backprojection_t(im[None, None], a)[0, 0])
 
 
+def test_uneven_bin_counts_at_wave_boundaries():
+    """Instances whose bin count sits either side of the forward's 63-bin
+    wavefronts (62, 63, 64, 126, 127) equal the per-instance ops bit for bit."""
+    from pridediff_amd.batched import UnevenBatch2d
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    projs, angs, imgs = [], [], []
+    for i, d in enumerate((62, 63, 64, 126, 127)):
+        n = 40 + 4 * i
+        projs.append(projector2d(n, d, 0.7 * 4, 1.2858 * 2, 595.0, 1085.6, scan_type="arc").cuda())
+        v = 20 + 3 * i
+        angs.append(torch.from_numpy((np.arange(v) * (2 * np.pi / v) + 0.1).astype(np.float32)).cuda())
+        imgs.append(torch.from_numpy(ellipses2d(n, 1, seed=300 + i, scale=0.04))[0].cuda())
+    ub = UnevenBatch2d(projs, angs)
+    with torch.no_grad():
+        sinos = ub.unpack_sinos(ub.projection(ub.pack_images(imgs)))
+        for P, a, im, s in zip(projs, angs, imgs, sinos):
+            assert torch.equal(s, P.projection(im[None, None], a)[0, 0])
+
+
 def test_uneven_graph_replay_and_autotune():
     from pridediff_amd.autotune import tune_uneven
     from pridediff_amd.batched import GraphStep, UnevenBatch2d, data_consistency_grad

@@ -68,10 +68,12 @@ def test_empty_batch_and_no_views(rt, geom):
 
 @pytest.mark.parametrize("geom", ["para", "flat", "arc"])
 @pytest.mark.parametrize("w,h,d,v", [(37, 29, 53, 7), (1, 1, 1, 1), (5, 64, 3, 4), (64, 5, 9, 13),
-                                     (48, 48, 11, 6)])
+                                     (48, 48, 11, 6), (44, 40, 63, 5), (44, 40, 64, 5),
+                                     (30, 34, 126, 3), (30, 34, 127, 3)])
 def test_ragged_and_degenerate_sizes(rt, geom, w, h, d, v):
     """Odd and non-square images, one pixel, one bin, tall and wide images,
-    a detector far narrower than the image."""
+    a detector far narrower than the image, and bin counts either side of
+    the forward's 63-bin wavefronts."""
     g = _geom(geom, w, h, d, pix=2.0 if max(w, h) <= 5 else 1.0)
     ang = (np.arange(v) * (2 * np.pi / v) + 0.3).astype(np.float32)
     _check(rt, geom, g, ang, seed=w * 100 + h)
