@@ -27,7 +27,7 @@ CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
 
 
-KEY_VERSION = 3  # bumped whenever the kernels or the sweep change
+KEY_VERSION = 4  # bumped whenever the kernels or the sweep change
 
 
 def _device_key() -> str:
