@@ -130,7 +130,11 @@ __global__ __launch_bounds__(kThreads) void k_bins_split(
   for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
   bins_accumulate<BT, WM>(g, vg, e1, lines + static_cast<size_t>(v) * g.Lmax,
                           pint + static_cast<size_t>(v) * g.Lmax,
-                          (vg.xaxis ? xrow : xcol) + b0, bp_const(g), l0, l1, N, acc, edge_lane);
+                          (vg.xaxis ? xrow : xcol) +
+                              (image_quad_major<BT>()
+                                   ? static_cast<size_t>(b0 / 4) * g.W * g.H * 4
+                                   : static_cast<size_t>(b0)),
+                          bp_const(g), l0, l1, N, acc, edge_lane);
   if (edge_lane || k >= g.D) return;
 #pragma unroll
   for (int b = 0; b < BT; ++b)

@@ -137,6 +137,20 @@ __device__ __forceinline__ void setup2d_view(
 // x [B][H][W] -> xrow [H][W][Bp] (lines = rows) and
 //               xcol [W][H][Bp] with xcol[l][i] = x[H-1-i][l] (lines = columns,
 //               pixel index from the bottom, FanArcDisCPU2d.cpp:194).
+// Instance-interleaved image copies for tiles of 8+ instances are quad-major,
+// [Bp / 4][line][pixel][4]: the 4 float4 loads a lane makes for one pixel of
+// a 16-instance tile then each read consecutive pixels across the wavefront
+// (one 16-byte slice per pixel, the wave's pixels contiguous), instead of
+// every load touching every pixel record's cache line.  Tiles of 1-4 keep
+// [line][pixel][Bp].
+#ifndef DD2D_QM_IMAGE
+#define DD2D_QM_IMAGE 1
+#endif
+template <int BT>
+constexpr bool image_quad_major() {
+  return DD2D_QM_IMAGE && BT >= 8;
+}
+
 template <int BT>
 __device__ __forceinline__ void interleave_image_px(
     const Geo2& g, const float* __restrict__ x, float* __restrict__ xrow,
@@ -147,8 +161,21 @@ __device__ __forceinline__ void interleave_image_px(
     v[b] = (b0 + b < g.B)
                ? x[(static_cast<size_t>(b0 + b) * g.H + h) * g.W + w]
                : 0.0f;
-  store_tile<BT>(xrow + (static_cast<size_t>(h) * g.W + w) * g.Bp + b0, v);
-  store_tile<BT>(xcol + (static_cast<size_t>(w) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
+  if constexpr (image_quad_major<BT>()) {
+    const size_t plane = static_cast<size_t>(g.W) * g.H;  // pixels per quad plane
+    const size_t pr = static_cast<size_t>(h) * g.W + w;
+    const size_t pc = static_cast<size_t>(w) * g.H + (g.H - 1 - h);
+#pragma unroll
+    for (int q = 0; q < BT / 4; ++q) {
+      const float4 t = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+      const size_t pq = static_cast<size_t>(b0 / 4 + q) * plane;
+      *reinterpret_cast<float4*>(xrow + (pq + pr) * 4) = t;
+      *reinterpret_cast<float4*>(xcol + (pq + pc) * 4) = t;
+    }
+  } else {
+    store_tile<BT>(xrow + (static_cast<size_t>(h) * g.W + w) * g.Bp + b0, v);
+    store_tile<BT>(xcol + (static_cast<size_t>(w) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
+  }
 }
 
 // p [B][V][D] -> s [V][D][Bp] in walk order, optionally scaled by the FP
@@ -255,7 +282,8 @@ __device__ __forceinline__ void bins_accumulate(
     const int cs = max(p1.c, 0);
     const int ce = min(p2.c, N - 1);
     if (cs > ce) continue;
-    const float* __restrict__ line = src + static_cast<size_t>(l) * N * g.Bp;
+    const float* __restrict__ line = src + static_cast<size_t>(l) * N *
+                                               (image_quad_major<BT>() ? 4 : g.Bp);
     for (int c = cs; c <= ce; ++c) {
       float w = overlap_in_pixel(p1, p2, c) * lw;
       if constexpr (WM == W_BPW) {
@@ -264,7 +292,20 @@ __device__ __forceinline__ void bins_accumulate(
         w *= fbp_weight(g, vg, xc, yc);
       }
       float xv[BT];
-      load_tile<BT>(line + static_cast<size_t>(c) * g.Bp, xv);
+      if constexpr (image_quad_major<BT>()) {
+        const size_t plane4 = static_cast<size_t>(g.W) * g.H * 4;
+#pragma unroll
+        for (int q = 0; q < BT / 4; ++q) {
+          const float4 t = *reinterpret_cast<const float4*>(line + q * plane4 +
+                                                            static_cast<size_t>(c) * 4);
+          xv[4 * q] = t.x;
+          xv[4 * q + 1] = t.y;
+          xv[4 * q + 2] = t.z;
+          xv[4 * q + 3] = t.w;
+        }
+      } else {
+        load_tile<BT>(line + static_cast<size_t>(c) * g.Bp, xv);
+      }
 #pragma unroll
       for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
     }
