@@ -151,6 +151,46 @@ constexpr bool image_quad_major() {
   return DD2D_QM_IMAGE && BT >= 8;
 }
 
+// The sinogram copy of the sinogram -> image gathers likewise:
+// [Bp / 4][view][bin][4] for tiles of 8+ instances, [view][bin][Bp] below.
+#ifndef DD2D_QM_SINO
+#define DD2D_QM_SINO 1
+#endif
+template <int BT>
+constexpr bool sino_quad_major() {
+  return DD2D_QM_SINO && BT >= 8;
+}
+
+// View v of the sinogram copy for the tile starting at instance b0.
+template <int BT>
+__device__ __forceinline__ const float* sino_view(const Geo2& g, const float* sino, int v,
+                                                  int b0) {
+  if constexpr (sino_quad_major<BT>())
+    return sino + (static_cast<size_t>(b0 / 4) * g.V * g.D + static_cast<size_t>(v) * g.D) * 4;
+  else
+    return sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+}
+
+// The BT values of bin k of a view returned by sino_view.
+template <int BT>
+__device__ __forceinline__ void load_sino(const Geo2& g, const float* __restrict__ sv, int k,
+                                          float (&q)[BT]) {
+  if constexpr (sino_quad_major<BT>()) {
+    const size_t plane4 = static_cast<size_t>(g.V) * g.D * 4;
+#pragma unroll
+    for (int j = 0; j < BT / 4; ++j) {
+      const float4 t =
+          *reinterpret_cast<const float4*>(sv + j * plane4 + static_cast<size_t>(k) * 4);
+      q[4 * j] = t.x;
+      q[4 * j + 1] = t.y;
+      q[4 * j + 2] = t.z;
+      q[4 * j + 3] = t.w;
+    }
+  } else {
+    load_tile<BT>(sv + static_cast<size_t>(k) * g.Bp, q);
+  }
+}
+
 template <int BT>
 __device__ __forceinline__ void interleave_image_px(
     const Geo2& g, const float* __restrict__ x, float* __restrict__ xrow,
@@ -192,7 +232,15 @@ __device__ __forceinline__ void interleave_sino_bin(
     val[b] = (b0 + b < g.B)
                  ? p[(static_cast<size_t>(b0 + b) * g.V + v) * g.D + kk] * wk
                  : 0.0f;
-  store_tile<BT>(s + (static_cast<size_t>(v) * g.D + k) * g.Bp + b0, val);
+  if constexpr (sino_quad_major<BT>()) {
+#pragma unroll
+    for (int j = 0; j < BT / 4; ++j)
+      *reinterpret_cast<float4*>(
+          s + ((static_cast<size_t>(b0 / 4 + j) * g.V + v) * g.D + k) * 4) =
+          make_float4(val[4 * j], val[4 * j + 1], val[4 * j + 2], val[4 * j + 3]);
+  } else {
+    store_tile<BT>(s + (static_cast<size_t>(v) * g.D + k) * g.Bp + b0, val);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -393,11 +441,11 @@ __device__ __forceinline__ void pixel_one_view(
   }
   float gw = (WM == W_FP) ? pl : bp_const(g);
   if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
-  const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
   while (p1.c <= i) {
     const float w = overlap_in_pixel(p1, p2, i) * gw;
     float q[BT];
-    load_tile<BT>(sv + static_cast<size_t>(k) * g.Bp, q);
+    load_sino<BT>(g, sv, k, q);
 #pragma unroll
     for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
     if (++k >= g.D) break;
@@ -430,10 +478,10 @@ __device__ __forceinline__ void pixel_pair_view(
     p1 = p2;
     p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
   }
-  const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
   while (p1.c <= ilast) {
     float q[BT];
-    load_tile<BT>(sv + static_cast<size_t>(k) * g.Bp, q);
+    load_sino<BT>(g, sv, k, q);
     if (p1.c <= i0 && i0 <= p2.c) {
       const float w = overlap_in_pixel(p1, p2, i0) * gw0;
 #pragma unroll
@@ -660,13 +708,13 @@ __device__ __forceinline__ void pixels_tile_body(
         const int e1 = min(s.first[ll][j + 1] - 1, n - 2);
         float gw = (WM == W_FP) ? pint[static_cast<size_t>(v) * g.Lmax + lbase + ll] : kbp;
         if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
-        const float* __restrict__ sv = sino + static_cast<size_t>(v) * g.D * g.Bp + b0;
+        const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
         for (int e = e0; e <= e1; ++e) {
           const EdgePos p1{s.ec[ll][e], s.ef[ll][e]};
           const EdgePos p2{s.ec[ll][e + 1], s.ef[ll][e + 1]};
           const float w = overlap_in_pixel(p1, p2, i) * gw;
           float q[BT];
-          load_tile<BT>(sv + static_cast<size_t>(klo + e) * g.Bp, q);
+          load_sino<BT>(g, sv, klo + e, q);
 #pragma unroll
           for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
         }
