@@ -294,6 +294,9 @@ constexpr int kSplit = 8;
 // the same bits), and lane 63 only serves lane 62's upper edge.  Each edge is
 // evaluated once per line instead of twice.
 constexpr int kBinsPerWave = 63;
+#ifndef DD2D_SHARED_PIXEL_MIN_BT
+#define DD2D_SHARED_PIXEL_MIN_BT 2
+#endif
 
 __device__ __forceinline__ EdgePos edge_from_next_lane(EdgePos p) {
   constexpr int kWaveShl1 = 0x130;  // DPP wave_shl:1: lane i reads lane i + 1
@@ -326,20 +329,9 @@ __device__ __forceinline__ void bins_accumulate(
     if constexpr (WM == W_FP) lw_next = pv[min(l + 1, l1 - 1)];
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_from_next_lane(p1);
-    if (edge_lane) continue;
-    const int cs = max(p1.c, 0);
-    const int ce = min(p2.c, N - 1);
-    if (cs > ce) continue;
     const float* __restrict__ line = src + static_cast<size_t>(l) * N *
                                                (image_quad_major<BT>() ? 4 : g.Bp);
-    for (int c = cs; c <= ce; ++c) {
-      float w = overlap_in_pixel(p1, p2, c) * lw;
-      if constexpr (WM == W_BPW) {
-        float xc, yc;
-        pixel_centre(g, xaxis, l, c, &xc, &yc);
-        w *= fbp_weight(g, vg, xc, yc);
-      }
-      float xv[BT];
+    auto load_px = [&](int c, float (&xv)[BT]) {
       if constexpr (image_quad_major<BT>()) {
         const size_t plane4 = static_cast<size_t>(g.W) * g.H * 4;
 #pragma unroll
@@ -354,8 +346,69 @@ __device__ __forceinline__ void bins_accumulate(
       } else {
         load_tile<BT>(line + static_cast<size_t>(c) * g.Bp, xv);
       }
+    };
+    auto weight = [&](int c) {
+      float w = overlap_in_pixel(p1, p2, c) * lw;
+      if constexpr (WM == W_BPW) {
+        float xc, yc;
+        pixel_centre(g, xaxis, l, c, &xc, &yc);
+        w *= fbp_weight(g, vg, xc, yc);
+      }
+      return w;
+    };
+    const int cs = max(p1.c, 0);
+    if constexpr (BT >= DD2D_SHARED_PIXEL_MIN_BT) {
+      // A bin's last pixel is the pixel of its upper edge, which is the next
+      // bin's first pixel: every lane (lane 63 too) loads its first pixel and
+      // takes its last one from the next lane, so each pixel of the line is
+      // loaded once per wavefront instead of once per bin that reaches it.
+      float x0[BT];
+      if (p1.c <= N - 1) {
+        load_px(cs, x0);
+      } else {
 #pragma unroll
-      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
+        for (int b = 0; b < BT; ++b) x0[b] = 0.0f;
+      }
+      float xn[BT];
+#pragma unroll
+      for (int b = 0; b < BT; ++b)
+        xn[b] = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                                              0, __builtin_bit_cast(int, x0[b]), 0x130, 0xf,
+                                              0xf, false));
+      if (edge_lane) continue;
+      const int ce = min(p2.c, N - 1);
+      if (cs > ce) continue;
+      {
+        const float w = weight(cs);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, x0[b], acc[b]);
+      }
+      if (ce > cs) {
+        for (int c = cs + 1; c < ce; ++c) {  // footprints wider than two pixels
+          float xv[BT];
+          load_px(c, xv);
+          const float w = weight(c);
+#pragma unroll
+          for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
+        }
+        // the next lane's first pixel is this bin's last whenever the upper
+        // edge lies inside the image (p2.c == ce); past the image, load it
+        if (p2.c != ce) load_px(ce, xn);
+        const float w = weight(ce);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xn[b], acc[b]);
+      }
+    } else {
+      if (edge_lane) continue;
+      const int ce = min(p2.c, N - 1);
+      if (cs > ce) continue;
+      for (int c = cs; c <= ce; ++c) {
+        const float w = weight(c);
+        float xv[BT];
+        load_px(c, xv);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, xv[b], acc[b]);
+      }
     }
   }
 }
