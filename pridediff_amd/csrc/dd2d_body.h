@@ -143,22 +143,16 @@ __device__ __forceinline__ void setup2d_view(
 // (one 16-byte slice per pixel, the wave's pixels contiguous), instead of
 // every load touching every pixel record's cache line.  Tiles of 1-4 keep
 // [line][pixel][Bp].
-#ifndef DD2D_QM_IMAGE
-#define DD2D_QM_IMAGE 1
-#endif
 template <int BT>
 constexpr bool image_quad_major() {
-  return DD2D_QM_IMAGE && BT >= 8;
+  return BT >= 8;
 }
 
 // The sinogram copy of the sinogram -> image gathers likewise:
 // [Bp / 4][view][bin][4] for tiles of 8+ instances, [view][bin][Bp] below.
-#ifndef DD2D_QM_SINO
-#define DD2D_QM_SINO 1
-#endif
 template <int BT>
 constexpr bool sino_quad_major() {
-  return DD2D_QM_SINO && BT >= 8;
+  return BT >= 8;
 }
 
 // View v of the sinogram copy for the tile starting at instance b0.
@@ -294,9 +288,8 @@ constexpr int kSplit = 8;
 // the same bits), and lane 63 only serves lane 62's upper edge.  Each edge is
 // evaluated once per line instead of twice.
 constexpr int kBinsPerWave = 63;
-#ifndef DD2D_SHARED_PIXEL_MIN_BT
-#define DD2D_SHARED_PIXEL_MIN_BT 2
-#endif
+// Tiles from this size up share pixels between forward lanes (below).
+constexpr int kSharedPixelMinBT = 2;
 
 __device__ __forceinline__ EdgePos edge_from_next_lane(EdgePos p) {
   constexpr int kWaveShl1 = 0x130;  // DPP wave_shl:1: lane i reads lane i + 1
@@ -357,7 +350,7 @@ __device__ __forceinline__ void bins_accumulate(
       return w;
     };
     const int cs = max(p1.c, 0);
-    if constexpr (BT >= DD2D_SHARED_PIXEL_MIN_BT) {
+    if constexpr (BT >= kSharedPixelMinBT) {
       // A bin's last pixel is the pixel of its upper edge, which is the next
       // bin's first pixel: every lane (lane 63 too) loads its first pixel and
       // takes its last one from the next lane, so each pixel of the line is
