@@ -174,7 +174,8 @@ size_t dd2d_batch_device_bytes(const dd2d_batch* b);
 /* Persistent grid size in workgroups (0 = 8 per compute unit). */
 void dd2d_batch_set_grid(dd2d_batch* b, int workgroups);
 /* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL, DD_ADJ_TILE16 or
- * DD_ADJ_QUAD. */
+ * DD_ADJ_QUAD (in a batch: one thread per 4x4 pixel block, four lines of
+ * four pixels; the default). */
 void dd2d_batch_set_variant(dd2d_batch* b, int variant);
 dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes,
                            void* stream);

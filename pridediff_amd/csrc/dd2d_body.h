@@ -624,6 +624,134 @@ __device__ __forceinline__ void quads_body(
             out[(static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc] = acc[dr][dc][b];
 }
 
+// Contributions of view v to the run of NP consecutive pixels ib .. ib+NP-1
+// of one line, of which positions pf .. pl are real pixels: one first-bin
+// search for pixel ib + pf, then one walk over the bins that reach any of
+// them, each sinogram tile loaded once.  Per pixel the same bins in the same
+// order as pixel_one_view: equal bit for bit.
+template <int BT, int WM, int NP>
+__device__ __forceinline__ void pixel_run_view(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const LineGeo& lg, float pl, const float* __restrict__ sino, int ib, int pf, int plst,
+    int N, int b0, const float (&gw)[NP], float (&acc)[NP][BT]) {
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  const int ifirst = ib + pf, ilast = ib + plst;
+  int k = min(max(guess_bin_fast(g, vg, lg.a0 + (ifirst + 0.5) * static_cast<double>(pl)), 0),
+              g.D - 1);
+  EdgePos p1 = edge_pos(ev[k], lg.a0, lg.inv_p, N);
+  while (k > 0 && p1.c >= ifirst) p1 = edge_pos(ev[--k], lg.a0, lg.inv_p, N);
+  EdgePos p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  while (p2.c < ifirst) {
+    if (++k >= g.D) return;
+    p1 = p2;
+    p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  }
+  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
+  while (p1.c <= ilast) {
+    float q[BT];
+    load_sino<BT>(g, sv, k, q);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int i = ib + p;
+      if (p >= pf && p <= plst && p1.c <= i && i <= p2.c) {
+        const float w = overlap_in_pixel(p1, p2, i) * gw[p];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) acc[p][b] = fmaf(w, q[b], acc[p][b]);
+      }
+    }
+    if (++k >= g.D) break;
+    p1 = p2;
+    if (p1.c > ilast) break;
+    p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  }
+}
+
+// sinogram -> image for the Q x Q pixel block at (r0, c0), all views: per
+// view the block is Q lines of Q consecutive pixels whichever the axis (rows
+// over columns c0 .. c0+Q-1, or columns read bottom-up).  Equal bit for bit
+// to pixels_body on each pixel.
+template <int BT, int WM, int Q>
+__device__ __forceinline__ void blocks_body(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ sino, float* __restrict__ out, int r0, int c0, int b0) {
+  const int nr = min(Q, g.H - r0), nc = min(Q, g.W - c0);
+  // pixel centres once, before the view loop, as pixels_body takes them
+  float xc[Q][Q], yc[Q][Q];
+  if constexpr (WM == W_BPW) {
+#pragma unroll
+    for (int dr = 0; dr < Q; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < Q; ++dc)
+        pixel_centre(g, 1, min(r0 + dr, g.H - 1), min(c0 + dc, g.W - 1), &xc[dr][dc],
+                     &yc[dr][dc]);
+  }
+  float acc[Q][Q][BT];  // [row offset][column offset][instance]
+#pragma unroll
+  for (int dr = 0; dr < Q; ++dr)
+#pragma unroll
+    for (int dc = 0; dc < Q; ++dc)
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc[dr][dc][b] = 0.0f;
+  const float kbp = bp_const(g);
+  for (int v = 0; v < g.V; ++v) {
+    const ViewGeo vg = views[v];
+    const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
+    const float* __restrict__ pv = pint + static_cast<size_t>(v) * g.Lmax;
+    if (vg.xaxis) {
+      // lines = image rows r0 + dr; run position p = column c0 + p
+#pragma unroll
+      for (int dr = 0; dr < Q; ++dr) {
+        if (dr >= nr) break;
+        const int l = r0 + dr;
+        const float pl = pv[l];
+        float gw[Q];
+#pragma unroll
+        for (int p = 0; p < Q; ++p) {
+          gw[p] = (WM == W_FP) ? pl : kbp;
+          if constexpr (WM == W_BPW) gw[p] *= fbp_weight(g, vg, xc[dr][p], yc[dr][p]);
+        }
+        pixel_run_view<BT, WM, Q>(g, vg, v, edges, lv[l], pl, sino, c0, 0, nc - 1, g.W, b0, gw,
+                                  acc[dr]);
+      }
+    } else {
+      // lines = image columns c0 + dc; pixel i = H - 1 - r, so run position p
+      // (pixel H - r0 - Q + p) is row r0 + Q - 1 - p, real when p >= Q - nr
+      float run[Q][BT];
+#pragma unroll
+      for (int dc = 0; dc < Q; ++dc) {
+        if (dc >= nc) break;
+        const int l = c0 + dc;
+        const float pl = pv[l];
+        float gw[Q];
+#pragma unroll
+        for (int p = 0; p < Q; ++p) {
+          gw[p] = (WM == W_FP) ? pl : kbp;
+          if constexpr (WM == W_BPW)
+            gw[p] *= fbp_weight(g, vg, xc[Q - 1 - p][dc], yc[Q - 1 - p][dc]);
+#pragma unroll
+          for (int b = 0; b < BT; ++b) run[p][b] = acc[Q - 1 - p][dc][b];
+        }
+        pixel_run_view<BT, WM, Q>(g, vg, v, edges, lv[l], pl, sino, g.H - r0 - Q, Q - nr, Q - 1,
+                                  g.H, b0, gw, run);
+#pragma unroll
+        for (int p = 0; p < Q; ++p)
+#pragma unroll
+          for (int b = 0; b < BT; ++b) acc[Q - 1 - p][dc][b] = run[p][b];
+      }
+    }
+  }
+#pragma unroll
+  for (int dr = 0; dr < Q; ++dr)
+#pragma unroll
+    for (int dc = 0; dc < Q; ++dc)
+      if (dr < nr && dc < nc)
+#pragma unroll
+        for (int b = 0; b < BT; ++b)
+          if (b0 + b < g.B)
+            out[(static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc] = acc[dr][dc][b];
+}
+
 // sinogram -> image for one pixel, all views, each view found independently.
 template <int BT, int WM>
 __device__ __forceinline__ void pixels_body(

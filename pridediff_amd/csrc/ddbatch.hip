@@ -72,11 +72,19 @@ enum Phase {
   kLayoutSino = 2,
   kGatherPixels = 3,
   kReduceBins = 4,
-  kGatherQuads = 5,  // sinogram -> image with one thread per 2 x 2 pixels
+  kGatherQuads = 5,  // sinogram -> image with one thread per 4 x 4 pixels
   kPhases = 6
 };
 
 constexpr int kBT = 1;  // one geometry per instance
+
+// Block items of the DD_ADJ_QUAD variant: one thread per kBlockQ x kBlockQ
+// pixel block (one first-bin search and one bin walk per line serve kBlockQ
+// pixels), 16 x 16 threads.  Measured on 64 uneven instances: adjoint 17.4 /
+// 16.4 / 14.7 / 17.6 / 21.0 ms for blocks of 2 / 3 / 4 / 5 / 6 (register
+// pressure past 4).
+constexpr int kBlockQ = 4;
+constexpr int kQuadTile = 16 * kBlockQ;  // pixels per quad item edge
 
 __device__ __forceinline__ int find_inst(const char* __restrict__ mem, int n, int phase,
                                          int item) {
@@ -248,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_pixels(
   }
 }
 
-// Quad items: (instance, 32 x 32 pixel tile), one thread per 2 x 2 pixels.
+// Block items: (instance, 64 x 64 pixel tile), one thread per 4 x 4 pixels.
 template <int WM>
 __global__ __launch_bounds__(kThreads) void k_batch_quads(
     char* __restrict__ mem, int n_inst, int total, float* __restrict__ out) {
@@ -256,15 +264,15 @@ __global__ __launch_bounds__(kThreads) void k_batch_quads(
     const int i = find_inst(mem, n_inst, kGatherQuads, item);
     const BatchInst d = load_inst(mem, i);
     const int local = item - item0_of(mem, i, kGatherQuads);
-    const int tx = (d.g.W + 2 * kTile - 1) / (2 * kTile);
-    const int ty = (d.g.H + 2 * kTile - 1) / (2 * kTile);
+    const int tx = (d.g.W + kQuadTile - 1) / kQuadTile;
+    const int ty = (d.g.H + kQuadTile - 1) / kQuadTile;
     const int t = local % (tx * ty);
     const int b0 = (local / (tx * ty)) * kBT;
-    const int c0 = (t % tx) * (2 * kTile) + 2 * (threadIdx.x % kTile);
-    const int r0 = (t / tx) * (2 * kTile) + 2 * (threadIdx.x / kTile);
+    const int c0 = (t % tx) * kQuadTile + kBlockQ * (threadIdx.x % kTile);
+    const int r0 = (t / tx) * kQuadTile + kBlockQ * (threadIdx.x / kTile);
     if (c0 < d.g.W && r0 < d.g.H)
-      quads_body<kBT, WM>(d.g, d.views, d.edges, d.lines, d.pint, d.sino, out + d.img_off, r0,
-                          c0, b0);
+      blocks_body<kBT, WM, kBlockQ>(d.g, d.views, d.edges, d.lines, d.pint, d.sino,
+                                    out + d.img_off, r0, c0, b0);
   }
 }
 
@@ -281,7 +289,7 @@ struct dd2d_batch {
   size_t total_views = 0;
   int total[kPhases] = {0, 0, 0, 0, 0, 0};
   int grid = 0;                  // persistent workgroups (0 = automatic)
-  int variant = DD_ADJ_QUAD;     // sinogram -> image kernel (per pixel, 16x16 tiles, 2x2 quads)
+  int variant = DD_ADJ_QUAD;     // sinogram -> image kernel (per pixel, 16x16 tiles, 4x4 blocks)
   int cus = 0;
   char* dev = nullptr;           // device memory given to setup()
 };
@@ -371,7 +379,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     d.nitems[kGatherBins] = kSplit * g.V * ((g.D + kBinsPerWave - 1) / kBinsPerWave) * tiles;
     d.nitems[kReduceBins] = g.V * ((g.D + kThreads - 1) / kThreads) * it.batch;
     d.nitems[kGatherQuads] =
-        ((g.W + 2 * kTile - 1) / (2 * kTile)) * ((g.H + 2 * kTile - 1) / (2 * kTile)) * tiles;
+        ((g.W + kQuadTile - 1) / kQuadTile) * ((g.H + kQuadTile - 1) / kQuadTile) * tiles;
     d.nitems[kLayoutSino] = g.V * tiles;
     d.nitems[kGatherPixels] = ((g.W + kTile - 1) / kTile) * ((g.H + kTile - 1) / kTile) * tiles;
     for (int ph = 0; ph < kPhases; ++ph) {
