@@ -501,6 +501,91 @@ __device__ __forceinline__ void pixel_one_view(
   }
 }
 
+__device__ __forceinline__ int int_from_next_lane(int x, int old) {
+  constexpr int kWaveShl1 = 0x130;  // DPP wave_shl:1: lane i reads lane i + 1
+  return __builtin_amdgcn_update_dpp(old, x, kWaveShl1, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ float float_from_next_lane(float x) {
+  return __builtin_bit_cast(float, int_from_next_lane(__builtin_bit_cast(int, x), 0));
+}
+
+// Contribution of view v to pixel i of a line that the whole wavefront
+// shares, lane j holding pixel i0 + j (every lane calls this together: the
+// DPP reads the next lane).  A pixel's last bin is, away from the detector's
+// ends, the bin in which the next pixel's walk starts; the next lane has
+// found that bin, loaded its sinogram tile and evaluated its upper edge on
+// the same line, so this lane takes the three from it instead of loading
+// them again.  `exists`: pixel i is in the line (the lane then also serves
+// the lane before it); `own`: the lane accumulates pixel i.  Same bins, same
+// order, same values as pixel_one_view: equal bit for bit.
+template <int BT, int WM>
+__device__ __forceinline__ void pixel_view_shared(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const LineGeo& lg, float pl, const float* __restrict__ sino, int i, bool exists, bool own,
+    int N, int b0, float xcw, float ycw, float (&acc)[BT]) {
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  // first bin as pixel_one_view finds it; k = -1 when the pixel gets none
+  int k = -1;
+  EdgePos p1{0, 0.0f}, p2{0, 0.0f};
+  if (exists) {
+    int kk = min(max(guess_bin_fast(g, vg, lg.a0 + (i + 0.5) * static_cast<double>(pl)), 0),
+                 g.D - 1);
+    p1 = edge_pos(ev[kk], lg.a0, lg.inv_p, N);
+    while (kk > 0 && p1.c >= i) p1 = edge_pos(ev[--kk], lg.a0, lg.inv_p, N);
+    p2 = edge_pos(ev[kk + 1], lg.a0, lg.inv_p, N);
+    bool found = true;
+    while (p2.c < i) {
+      if (++kk >= g.D) {
+        found = false;
+        break;
+      }
+      p1 = p2;
+      p2 = edge_pos(ev[kk + 1], lg.a0, lg.inv_p, N);
+    }
+    if (found && p1.c <= i) k = kk;
+  }
+  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
+  float q[BT];
+  if (k >= 0) {
+    load_sino<BT>(g, sv, k, q);
+  } else {
+#pragma unroll
+    for (int b = 0; b < BT; ++b) q[b] = 0.0f;
+  }
+  // the next lane's first bin, its tile and its upper edge (lane 63: none)
+  const int kn = int_from_next_lane(k, -1);
+  float qn[BT];
+#pragma unroll
+  for (int b = 0; b < BT; ++b) qn[b] = float_from_next_lane(q[b]);
+  const EdgePos pn = edge_from_next_lane(p2);
+  if (!own || k < 0) return;
+  float gw = (WM == W_FP) ? pl : bp_const(g);
+  if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
+  {
+    const float w = overlap_in_pixel(p1, p2, i) * gw;
+#pragma unroll
+    for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
+  }
+  while (true) {
+    if (++k >= g.D) break;
+    p1 = p2;
+    if (p1.c > i) break;
+    if (k == kn) {
+      p2 = pn;
+      const float w = overlap_in_pixel(p1, p2, i) * gw;
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, qn[b], acc[b]);
+    } else {
+      p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+      const float w = overlap_in_pixel(p1, p2, i) * gw;
+      load_sino<BT>(g, sv, k, q);
+#pragma unroll
+      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
+    }
+  }
+}
+
 // Contributions of view v to the two consecutive pixels i0, i0 + 1 of one
 // line (i0 + 1 only when has1): one first-bin search for i0, then one walk
 // over the bins that reach either pixel, each sinogram tile loaded once.  A
