@@ -469,11 +469,9 @@ def main():
     rt.profile_enable(False)
     fp_ms, bp_ms = fp_ms / max(fp_n, 1), bp_ms / max(bp_n, 1)
     b_alg = 4.0 * V * (N * N + D) * B  # bytes per operator launch over the batch
-    # the adjoint kernel of the chosen variant (the axis-run variant is one
-    # launch per run of views; its time and traffic are summed per operator)
-    adj = {rt.ADJ_RUNS: "k_pixels_runs", rt.ADJ_QUAD: "k_pixels_quad",
-           rt.ADJ_TILE16: "k_pixels_tile", rt.ADJ_TILE8: "k_pixels_tile"}.get(BP.variant,
-                                                                              "k_pixels")
+    # the adjoint kernel of the chosen variant
+    adj = {rt.ADJ_QUAD: "k_pixels_quad", rt.ADJ_TILE16: "k_pixels_tile",
+           rt.ADJ_TILE8: "k_pixels_tile"}.get(BP.variant, "k_pixels")
     kernels = {
         "k_bins_split + k_bins_reduce (A, image->sino)":
             {"ms": fp_ms, "GB/s": b_alg / (fp_ms * 1e-3) / 1e9},

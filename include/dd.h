@@ -127,16 +127,12 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
                         int views, void* geo, size_t geo_bytes, void* stream);
 /* Kernel variants of the sinogram -> image gathers (all exact and equal):
  * each pixel finds its bins on its own; a 16x16 / 8x8 pixel tile stages its
- * lines' bin-edge positions per view in LDS; one thread walks the bins of
- * a 2x2 pixel block once for its two lines of two pixels; or the views are
- * taken in runs of one common axis, one launch per run (up to five), with a
- * wavefront on 64 consecutive pixels of one line, where each pixel takes its
- * last bin from the next pixel's thread instead of loading it again. */
+ * lines' bin-edge positions per view in LDS; or one thread walks the bins of
+ * a 2x2 pixel block once for its two lines of two pixels. */
 #define DD_ADJ_PER_PIXEL 0
 #define DD_ADJ_TILE16 1
 #define DD_ADJ_TILE8 2
 #define DD_ADJ_QUAD 3
-#define DD_ADJ_RUNS 4
 
 /* `tile` = instances sharing one geometry evaluation (1, 2, 4, 8 or 16;
  * 0 = automatic); it must be the same in both calls.  `variant` selects the

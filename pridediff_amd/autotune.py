@@ -3,9 +3,8 @@
 Sweeps the launch parameters of a planned operator on the problem it will
 run — the instance tile (how many instances share one geometry evaluation,
 1-16) and the sinogram -> image kernel variant (per pixel, 16x16 or 8x8
-tiles with LDS edge positions, one thread per 2x2 pixels, or passes over
-runs of views of one common axis) of a uniform batch, the persistent grid
-size and
+tiles with LDS edge positions, or one thread per 2x2 pixels) of a uniform
+batch, the persistent grid size and
 kernel variant of an uneven batch —
 times one forward projection plus one adjoint per candidate with HIP events,
 keeps the fastest, and remembers it in a JSON cache keyed by the device and
@@ -77,7 +76,7 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
     """Fastest ``BatchedProjector2d`` (tile, kernel variant) for this
     projector, angle set and batch."""
     if variants is None:
-        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD, rt.ADJ_RUNS)
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD)
     opts = [round(float(v), 6) for v in projector.host_options().tolist()]
     key = f"{_device_key()}|uniform|{projector.scan_type}|{opts}|V={angles.numel()}|B={batch}"
     table = _load() if use_cache else {}

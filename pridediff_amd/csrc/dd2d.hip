@@ -14,11 +14,7 @@
 //                       chunk of image lines gathers along its lines (no
 //                       atomics); k_bins_reduce adds the chunks in order.
 //      k_pixels         sinogram -> image ops: one thread per pixel gathers
-//                       over every view (no atomics); or k_pixels_runs, one
-//                       launch per run of views of one common axis, whose
-//                       wavefronts lie along that axis's lines so neighbouring
-//                       pixels share their boundary bins (k_axis_runs finds
-//                       the runs once per geometry).
+//                       over every view (no atomics).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,7 +53,7 @@ int compute_units() {
 }
 
 int adjoint_variant(int requested, const Geo2& g) {
-  if (requested < DD_ADJ_PER_PIXEL || requested > DD_ADJ_RUNS) requested = g_adjoint_variant;
+  if (requested < DD_ADJ_PER_PIXEL || requested > DD_ADJ_QUAD) requested = g_adjoint_variant;
   if (requested >= DD_ADJ_PER_PIXEL) return requested;
   const long long quad_groups = 1LL * ((g.W + 31) / 32) * ((g.H + 31) / 32) * (g.Bp / g.BT);
   return (g.BT <= 4 && quad_groups >= 4LL * compute_units()) ? DD_ADJ_QUAD : DD_ADJ_PER_PIXEL;
@@ -236,97 +232,6 @@ __global__ __launch_bounds__(T * T) void k_pixels_tile(
                               blockIdx.y, blockIdx.z * BT, lds);
 }
 
-// Variant DD_ADJ_RUNS: the views fall into runs of one common axis (a full
-// rotation has at most five: the axis turns every quarter turn).  Pass p
-// (p < kRunPasses - 1) takes run p, the last pass every view left; a
-// wavefront holds kRunPixels consecutive pixels of one line of its pass's
-// axis, so on every view of a pure run neighbouring lanes share their
-// boundary bins (pixel_view_shared).  Per pixel the views are still taken
-// in order 0 .. V - 1 and the running sums travel between passes through
-// memory unchanged: equal bit for bit to k_pixels.
-constexpr int kRunPasses = 5;
-// Pixels per wavefront (lane 63 loads its own last bin).
-constexpr int kRunPixels = 64;
-
-// runs[p] = first view of run p (p < kRunPasses), V past the last run.
-__global__ __launch_bounds__(64) void k_axis_runs(Geo2 g, const ViewGeo* __restrict__ views,
-                                                  int* __restrict__ runs) {
-  int found = 0;  // wave-uniform
-  for (int v0 = 0; v0 < g.V && found < kRunPasses; v0 += 64) {
-    const int v = v0 + static_cast<int>(threadIdx.x);
-    const bool starts = v < g.V && (v == 0 || views[v].xaxis != views[v - 1].xaxis);
-    const unsigned long long m = __ballot(starts);
-    const int rank = found + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                                 static_cast<unsigned>(m >> 32),
-                                 __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0u)));
-    if (starts && rank < kRunPasses) runs[rank] = v;
-    found += __popcll(m);
-  }
-  if (static_cast<int>(threadIdx.x) >= found && threadIdx.x < kRunPasses) runs[threadIdx.x] = g.V;
-}
-
-// Waves one pass needs at most: the larger of both line orientations.
-inline long long run_pass_waves(const Geo2& g) {
-  const long long xw = 1LL * g.H * ((g.W + kRunPixels - 1) / kRunPixels);
-  const long long yw = 1LL * g.W * ((g.H + kRunPixels - 1) / kRunPixels);
-  return std::max(xw, yw);
-}
-
-// One pass of DD_ADJ_RUNS over instance tile blockIdx.y.  part = running
-// sums between passes, [tile][H][W][BT].
-template <int BT, int WM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_pixels_runs(
-    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
-    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const int* __restrict__ runs, int pass, const float* __restrict__ sino,
-    float* __restrict__ part, float* __restrict__ out) {
-  const int va = runs[pass];
-  const int vb = pass + 1 < kRunPasses ? runs[pass + 1] : g.V;
-  if (va >= vb) return;
-  const int m = views[va].xaxis;         // 1: lines are image rows
-  const int N = m ? g.W : g.H;           // pixels per line
-  const int nl = m ? g.H : g.W;          // lines
-  const int wave = __builtin_amdgcn_readfirstlane(
-      static_cast<int>(blockIdx.x * (kThreads / 64) + threadIdx.x / 64));
-  const int line = wave % nl, seg = wave / nl;
-  if (seg * kRunPixels >= N) return;     // whole wavefront past the line
-  const int lane = static_cast<int>(threadIdx.x % 64);
-  const int i = seg * kRunPixels + lane;
-  const bool exists = i < N, own = exists && lane < kRunPixels;
-  const int ic = min(i, N - 1);
-  const int r = m ? line : g.H - 1 - ic, c = m ? ic : line;
-  const int b0 = blockIdx.y * BT;
-  float xcw = 0.0f, ycw = 0.0f;
-  if constexpr (WM == W_BPW) pixel_centre(g, 1, r, c, &xcw, &ycw);
-  float* __restrict__ pp =
-      part + ((static_cast<size_t>(blockIdx.y) * g.H + r) * g.W + c) * BT;
-  float acc[BT];
-  if (va > 0 && own) {
-    load_tile<BT>(pp, acc);
-  } else {
-#pragma unroll
-    for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
-  }
-  for (int v = va; v < vb; ++v) {
-    const ViewGeo vg = views[v];
-    if (vg.xaxis == m) {
-      const size_t at = static_cast<size_t>(v) * g.Lmax + line;
-      pixel_view_shared<BT, WM>(g, vg, v, edges, lines[at], pint[at], sino, i, exists, own, N,
-                                b0, xcw, ycw, acc);
-    } else if (own) {
-      pixel_one_view<BT, WM>(g, vg, v, edges, lines, pint, sino, r, c, b0, xcw, ycw, acc);
-    }
-  }
-  if (!own) return;
-  if (vb < g.V) {
-    store_tile<BT>(pp, acc);
-  } else {
-#pragma unroll
-    for (int b = 0; b < BT; ++b)
-      if (b0 + b < g.B) out[(static_cast<size_t>(b0 + b) * g.H + r) * g.W + c] = acc[b];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // system matrix (COO, FP weights): count pass and fill pass of one walk
 // ---------------------------------------------------------------------------
@@ -381,8 +286,8 @@ __global__ __launch_bounds__(kThreads) void k_sysmatrix(
 // host side
 // ---------------------------------------------------------------------------
 struct Layout {
-  size_t views, edges, wfp, lines, pint, runs, geo_total;  // geometry tables
-  size_t buf0, buf1, part, data_total;                      // data buffers
+  size_t views, edges, wfp, lines, pint, geo_total;  // geometry tables
+  size_t buf0, buf1, part, data_total;                // data buffers
 };
 
 constexpr size_t kAlign = 256;
@@ -412,7 +317,6 @@ Layout make_layout(const Geo2& g, int op) {
   L.wfp = take(sizeof(float) * g.V * g.D);
   L.lines = take(sizeof(LineGeo) * g.V * g.Lmax);
   L.pint = take(sizeof(float) * g.V * g.Lmax);
-  L.runs = take(sizeof(int) * kRunPasses);
   L.geo_total = off;
   off = 0;
   if (to_sino(op)) {
@@ -422,8 +326,6 @@ Layout make_layout(const Geo2& g, int op) {
   } else {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * g.D * g.Bp);
     L.buf1 = L.buf0;
-    // running sums between the passes of DD_ADJ_RUNS
-    L.part = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
   }
   L.data_total = off;
   return L;
@@ -461,7 +363,6 @@ struct GeoPtrs {
   const float* wfp;
   const LineGeo* lines;
   const float* pint;
-  const int* runs;
 };
 
 GeoPtrs geo_ptrs(const void* geo, const Layout& L) {
@@ -470,14 +371,11 @@ GeoPtrs geo_ptrs(const void* geo, const Layout& L) {
           reinterpret_cast<const double*>(b + L.edges),
           reinterpret_cast<const float*>(b + L.wfp),
           reinterpret_cast<const LineGeo*>(b + L.lines),
-          reinterpret_cast<const float*>(b + L.pint),
-          reinterpret_cast<const int*>(b + L.runs)};
+          reinterpret_cast<const float*>(b + L.pint)};
 }
 
-// Geometry tables of one call; `runs`: also the axis-run table DD_ADJ_RUNS
-// reads (plans always build it, one-call ops only when they use it).
 void launch_setup(const Geo2& g, const float* angles, void* geo,
-                  const Layout& L, hipStream_t st, bool runs) {
+                  const Layout& L, hipStream_t st) {
   char* b = static_cast<char*>(geo);
   hipLaunchKernelGGL(k_setup2d, dim3(g.V), dim3(kThreads), 0, st, g, angles,
                      reinterpret_cast<ViewGeo*>(b + L.views),
@@ -485,10 +383,6 @@ void launch_setup(const Geo2& g, const float* angles, void* geo,
                      reinterpret_cast<float*>(b + L.wfp),
                      reinterpret_cast<LineGeo*>(b + L.lines),
                      reinterpret_cast<float*>(b + L.pint));
-  if (runs)
-    hipLaunchKernelGGL(k_axis_runs, dim3(1), dim3(64), 0, st, g,
-                       reinterpret_cast<const ViewGeo*>(b + L.views),
-                       reinterpret_cast<int*>(b + L.runs));
 }
 
 template <int BT>
@@ -533,22 +427,7 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       hipLaunchKernelGGL((k_interleave_sino<BT, false>), sgrid, dim3(kThreads),
                          0, st, g, G.views, G.wfp, input, buf0);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (variant == DD_ADJ_RUNS) {
-      auto* part = reinterpret_cast<float*>(ws + L.part);
-      dim3 grid(static_cast<unsigned>((run_pass_waves(g) + kThreads / 64 - 1) / (kThreads / 64)),
-                ntile);
-      for (int p = 0; p < kRunPasses; ++p) {
-        if (op == DD_PROJ_T)
-          hipLaunchKernelGGL((k_pixels_runs<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                             G.views, G.edges, G.lines, G.pint, G.runs, p, buf0, part, output);
-        else if (op == DD_BPROJ)
-          hipLaunchKernelGGL((k_pixels_runs<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                             G.views, G.edges, G.lines, G.pint, G.runs, p, buf0, part, output);
-        else
-          hipLaunchKernelGGL((k_pixels_runs<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                             G.views, G.edges, G.lines, G.pint, G.runs, p, buf0, part, output);
-      }
-    } else if (variant == DD_ADJ_QUAD) {
+    if (variant == DD_ADJ_QUAD) {
       dim3 grid((g.W + 2 * kTile - 1) / (2 * kTile), (g.H + 2 * kTile - 1) / (2 * kTile), ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_quad<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
@@ -653,7 +532,7 @@ extern "C" {
 const char* dd_version(void) { return "libdd 0.1 (gfx950)"; }
 
 void dd2d_set_adjoint_variant(int variant) {
-  g_adjoint_variant = (variant >= DD_ADJ_PER_PIXEL && variant <= DD_ADJ_RUNS) ? variant : -1;
+  g_adjoint_variant = (variant >= DD_ADJ_PER_PIXEL && variant <= DD_ADJ_QUAD) ? variant : -1;
 }
 
 void dd2d_set_instance_tile(int tile) {
@@ -680,9 +559,9 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
-  const int variant = adjoint_variant(-1, q);
-  launch_setup(q, angles, w, L, st, !to_sino(op) && variant == DD_ADJ_RUNS);
-  return run_tiles(q, op, variant, input, output, geo_ptrs(w, L), w + L.geo_total, L, st);
+  launch_setup(q, angles, w, L, st);
+  return run_tiles(q, op, adjoint_variant(-1, q), input, output, geo_ptrs(w, L),
+                   w + L.geo_total, L, st);
 }
 
 dd_status dd2d_run_views(int scan, int op, const dd_geom2d* g, const float* input,
@@ -700,7 +579,7 @@ dd_status dd2d_run_views(int scan, int op, const dd_geom2d* g, const float* inpu
   if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
-  launch_setup(q, angles, w, L, st, false);
+  launch_setup(q, angles, w, L, st);
   const GeoPtrs G = geo_ptrs(w, L);
   switch (q.BT) {
     case 1: return launch_views<1>(q, op, input, output, G, w + L.geo_total, L, st);
@@ -732,7 +611,7 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
   if (!angles || !fill_geo(&q, scan, g, 1, views, 1)) return DD_BAD_ARG;
   const Layout L = make_layout(q, DD_PROJ);
   if (!geo || geo_bytes < L.geo_total) return DD_NO_WORKSPACE;
-  launch_setup(q, angles, geo, L, static_cast<hipStream_t>(stream), true);
+  launch_setup(q, angles, geo, L, static_cast<hipStream_t>(stream));
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
 }
 

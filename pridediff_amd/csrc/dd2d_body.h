@@ -252,27 +252,30 @@ __host__ __device__ __forceinline__ float bp_const(const Geo2& g) {
 
 // FBP distance weight of a pixel centre: arc 1/|S-P|^2
 // (FanArcDisCPU2d.cpp:27-29,507), flat (r/(r-d))^2 (FanFlatDisCPU2d.cpp:19-22).
+// Every product that feeds a sum here and in pixel_centre is an explicit
+// fmaf: left to contraction, which product the compiler fuses depends on
+// the code around the inlined call, and every kernel must get the same bits.
 __device__ __forceinline__ float fbp_weight(const Geo2& g, const ViewGeo& vg,
                                             float xc, float yc) {
   if (g.scan == DD_FAN_ARC) {
     const float ddx = vg.sxf - xc, ddy = vg.syf - yc;
-    return 1.0f / (ddx * ddx + ddy * ddy);
+    const float dy2 = ddy * ddy;
+    return 1.0f / fmaf(ddx, ddx, dy2);
   }
   const float r = g.sid;
-  const float d = (vg.sxf * xc + vg.syf * yc) / r;
+  const float syy = vg.syf * yc;
+  const float d = fmaf(vg.sxf, xc, syy) / r;
   return (r * r) / ((r - d) * (r - d));
 }
 
-// Pixel centre of (line, pixel) in world coordinates.
+// Pixel centre of (line, pixel) in world coordinates.  The offsets
+// (i - W/2 + 1/2 etc.) are exact in FP32, only the scaling rounds.
 __device__ __forceinline__ void pixel_centre(const Geo2& g, int xaxis, int l,
                                              int i, float* xc, float* yc) {
-  if (xaxis) {
-    *xc = (i - g.W * 0.5f + 0.5f) * g.dx + g.shx;
-    *yc = (g.H * 0.5f - l - 0.5f) * g.dy + g.shy;
-  } else {
-    *xc = (l - g.W * 0.5f + 0.5f) * g.dx + g.shx;
-    *yc = (i - g.H * 0.5f + 0.5f) * g.dy + g.shy;
-  }
+  const int cx = xaxis ? i : l, ry = xaxis ? l : i;
+  *xc = fmaf(static_cast<float>(cx) - g.W * 0.5f + 0.5f, g.dx, g.shx);
+  *yc = xaxis ? fmaf(g.H * 0.5f - static_cast<float>(ry) - 0.5f, g.dy, g.shy)
+              : fmaf(static_cast<float>(ry) - g.H * 0.5f + 0.5f, g.dy, g.shy);
 }
 
 // ---------------------------------------------------------------------------
@@ -501,88 +504,92 @@ __device__ __forceinline__ void pixel_one_view(
   }
 }
 
-__device__ __forceinline__ int int_from_next_lane(int x, int old) {
-  constexpr int kWaveShl1 = 0x130;  // DPP wave_shl:1: lane i reads lane i + 1
-  return __builtin_amdgcn_update_dpp(old, x, kWaveShl1, 0xf, 0xf, false);
-}
-
-__device__ __forceinline__ float float_from_next_lane(float x) {
-  return __builtin_bit_cast(float, int_from_next_lane(__builtin_bit_cast(int, x), 0));
-}
-
-// Contribution of view v to pixel i of a line that the whole wavefront
-// shares, lane j holding pixel i0 + j (every lane calls this together: the
-// DPP reads the next lane).  A pixel's last bin is, away from the detector's
-// ends, the bin in which the next pixel's walk starts; the next lane has
-// found that bin, loaded its sinogram tile and evaluated its upper edge on
-// the same line, so this lane takes the three from it instead of loading
-// them again.  `exists`: pixel i is in the line (the lane then also serves
-// the lane before it); `own`: the lane accumulates pixel i.  Same bins, same
-// order, same values as pixel_one_view: equal bit for bit.
-template <int BT, int WM>
-__device__ __forceinline__ void pixel_view_shared(
-    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
-    const LineGeo& lg, float pl, const float* __restrict__ sino, int i, bool exists, bool own,
-    int N, int b0, float xcw, float ycw, float (&acc)[BT]) {
-  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
-  // first bin as pixel_one_view finds it; k = -1 when the pixel gets none
-  int k = -1;
-  EdgePos p1{0, 0.0f}, p2{0, 0.0f};
-  if (exists) {
-    int kk = min(max(guess_bin_fast(g, vg, lg.a0 + (i + 0.5) * static_cast<double>(pl)), 0),
-                 g.D - 1);
-    p1 = edge_pos(ev[kk], lg.a0, lg.inv_p, N);
-    while (kk > 0 && p1.c >= i) p1 = edge_pos(ev[--kk], lg.a0, lg.inv_p, N);
-    p2 = edge_pos(ev[kk + 1], lg.a0, lg.inv_p, N);
-    bool found = true;
-    while (p2.c < i) {
-      if (++kk >= g.D) {
-        found = false;
-        break;
-      }
-      p1 = p2;
-      p2 = edge_pos(ev[kk + 1], lg.a0, lg.inv_p, N);
-    }
-    if (found && p1.c <= i) k = kk;
-  }
-  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
-  float q[BT];
-  if (k >= 0) {
-    load_sino<BT>(g, sv, k, q);
+// First-bin guess for the pixel centred at (xc, yc) from the view alone: the
+// axis coordinate of the ray from the source through the centre (the line's
+// pixel-edge map evaluated at the centre), in FP32.  It needs no per-line
+// table, so a view's search can be started before the previous view's walk.
+// Only a guess: the search below it is exact whatever it returns.
+__device__ __forceinline__ int guess_bin_centre(const Geo2& g, const ViewGeo& vg, float xc,
+                                                float yc) {
+  float a;
+  if (g.scan == DD_PARA) {
+    const float s = static_cast<float>(vg.s), c = static_cast<float>(vg.c);
+    a = vg.xaxis ? xc + yc * s * __builtin_amdgcn_rcpf(c) : yc + xc * c * __builtin_amdgcn_rcpf(s);
   } else {
-#pragma unroll
-    for (int b = 0; b < BT; ++b) q[b] = 0.0f;
+    a = vg.xaxis ? (vg.sxf * yc - vg.syf * xc) * __builtin_amdgcn_rcpf(yc - vg.syf)
+                 : (vg.syf * xc - vg.sxf * yc) * __builtin_amdgcn_rcpf(xc - vg.sxf);
   }
-  // the next lane's first bin, its tile and its upper edge (lane 63: none)
-  const int kn = int_from_next_lane(k, -1);
-  float qn[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b) qn[b] = float_from_next_lane(q[b]);
-  const EdgePos pn = edge_from_next_lane(p2);
-  if (!own || k < 0) return;
-  float gw = (WM == W_FP) ? pl : bp_const(g);
+  return min(max(guess_bin_fast(g, vg, static_cast<double>(a)), 0), g.D - 1);
+}
+
+// What a pixel's search in one view starts from: the line's pixel-edge map,
+// the guessed bin and that bin's two edges, all loaded ahead.
+struct PixViewStart {
+  LineGeo lg;
+  float pl;
+  int k0;
+  double ea, eb;  // edges k0 and k0 + 1
+};
+
+__device__ __forceinline__ PixViewStart pixel_view_start(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint, int r, int c, float xc,
+    float yc) {
+  PixViewStart s;
+  // a branch per axis, so a caller whose row r is wave-uniform gets scalar
+  // loads on row-axis views
+  if (vg.xaxis) {
+    s.lg = lines[static_cast<size_t>(v) * g.Lmax + r];
+    s.pl = pint[static_cast<size_t>(v) * g.Lmax + r];
+  } else {
+    s.lg = lines[static_cast<size_t>(v) * g.Lmax + c];
+    s.pl = pint[static_cast<size_t>(v) * g.Lmax + c];
+  }
+  s.k0 = guess_bin_centre(g, vg, xc, yc);
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  s.ea = ev[s.k0];
+  s.eb = ev[s.k0 + 1];
+  return s;
+}
+
+// pixel_one_view from a start loaded ahead: the same search (edge k is
+// P(k) = edge_pos(ev[k]); going down, the old lower edge becomes the upper
+// one), the same walk.  Same bins, same order, same values: equal bit for
+// bit.
+template <int BT, int WM>
+__device__ __forceinline__ void pixel_view_from(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const PixViewStart& s, const float* __restrict__ sino, int r, int c, int b0, float xcw,
+    float ycw, float (&acc)[BT]) {
+  const int i = vg.xaxis ? c : g.H - 1 - r;
+  const int N = vg.xaxis ? g.W : g.H;
+  const LineGeo lg = s.lg;
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  int k = s.k0;
+  EdgePos p1 = edge_pos(s.ea, lg.a0, lg.inv_p, N);
+  EdgePos p2 = edge_pos(s.eb, lg.a0, lg.inv_p, N);
+  while (k > 0 && p1.c >= i) {
+    p2 = p1;
+    p1 = edge_pos(ev[--k], lg.a0, lg.inv_p, N);
+  }
+  while (p2.c < i) {
+    if (++k >= g.D) return;
+    p1 = p2;
+    p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+  }
+  float gw = (WM == W_FP) ? s.pl : bp_const(g);
   if constexpr (WM == W_BPW) gw *= fbp_weight(g, vg, xcw, ycw);
-  {
+  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
+  while (p1.c <= i) {
     const float w = overlap_in_pixel(p1, p2, i) * gw;
+    float q[BT];
+    load_sino<BT>(g, sv, k, q);
 #pragma unroll
     for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
-  }
-  while (true) {
     if (++k >= g.D) break;
     p1 = p2;
     if (p1.c > i) break;
-    if (k == kn) {
-      p2 = pn;
-      const float w = overlap_in_pixel(p1, p2, i) * gw;
-#pragma unroll
-      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, qn[b], acc[b]);
-    } else {
-      p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
-      const float w = overlap_in_pixel(p1, p2, i) * gw;
-      load_sino<BT>(g, sv, k, q);
-#pragma unroll
-      for (int b = 0; b < BT; ++b) acc[b] = fmaf(w, q[b], acc[b]);
-    }
+    p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
   }
 }
 
@@ -837,19 +844,30 @@ __device__ __forceinline__ void blocks_body(
             out[(static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc] = acc[dr][dc][b];
 }
 
-// sinogram -> image for one pixel, all views, each view found independently.
+// sinogram -> image for one pixel, all views.  Each view's search is
+// started (line map, guess, first two edges loaded) before the previous
+// view's walk, so those loads overlap the walk's instead of opening the
+// view; per view the same bins in the same order as pixel_one_view.
 template <int BT, int WM>
 __device__ __forceinline__ void pixels_body(
     const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
     const float* __restrict__ sino, float* __restrict__ out, int r, int c, int b0) {
   float xcw = 0.0f, ycw = 0.0f;
-  if constexpr (WM == W_BPW) pixel_centre(g, 1, r, c, &xcw, &ycw);
+  pixel_centre(g, 1, r, c, &xcw, &ycw);
   float acc[BT];
 #pragma unroll
   for (int b = 0; b < BT; ++b) acc[b] = 0.0f;
-  for (int v = 0; v < g.V; ++v)
-    pixel_one_view<BT, WM>(g, views[v], v, edges, lines, pint, sino, r, c, b0, xcw, ycw, acc);
+  ViewGeo vg = views[0];
+  PixViewStart cur = pixel_view_start(g, vg, 0, edges, lines, pint, r, c, xcw, ycw);
+  for (int v = 0; v < g.V; ++v) {
+    const int vn = min(v + 1, g.V - 1);
+    const ViewGeo vgn = views[vn];
+    const PixViewStart nxt = pixel_view_start(g, vgn, vn, edges, lines, pint, r, c, xcw, ycw);
+    pixel_view_from<BT, WM>(g, vg, v, edges, cur, sino, r, c, b0, xcw, ycw, acc);
+    vg = vgn;
+    cur = nxt;
+  }
 #pragma unroll
   for (int b = 0; b < BT; ++b)
     if (b0 + b < g.B)

@@ -338,7 +338,7 @@ class Plan2d {
          int64_t batch, int64_t tile, int64_t variant)
       : scan_(scan), batch_(batch), tile_(static_cast<int>(tile)),
         variant_(static_cast<int>(variant)) {
-    TORCH_CHECK(variant >= -1 && variant <= DD_ADJ_RUNS, "Plan2d: unknown kernel variant");
+    TORCH_CHECK(variant >= -1 && variant <= DD_ADJ_QUAD, "Plan2d: unknown kernel variant");
     TORCH_CHECK(scan >= DD_PARA && scan <= DD_FAN_ARC, "Plan2d: 2D scan expected");
     TORCH_CHECK(angles.is_cuda() && angles.scalar_type() == torch::kFloat32 &&
                     angles.dim() == 1,
@@ -635,7 +635,6 @@ PYBIND11_MODULE(radontorch, m) {
   m.attr("ADJ_TILE16") = static_cast<int>(DD_ADJ_TILE16);
   m.attr("ADJ_TILE8") = static_cast<int>(DD_ADJ_TILE8);
   m.attr("ADJ_QUAD") = static_cast<int>(DD_ADJ_QUAD);
-  m.attr("ADJ_RUNS") = static_cast<int>(DD_ADJ_RUNS);
   m.attr("PROJ") = static_cast<int>(DD_PROJ);
   m.attr("PROJ_T") = static_cast<int>(DD_PROJ_T);
   m.attr("BPROJ") = static_cast<int>(DD_BPROJ);

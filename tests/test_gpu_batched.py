@@ -52,16 +52,16 @@ def test_adjoint_kernel_variants_agree_bitwise(scan, n):
         with torch.no_grad():
             outs[var] = (BP.projection_t(y), BP.backprojection(y), BP.backprojection_w(y))
     for var in (rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD):
-        for a, b in zip(outs[rt.ADJ_PER_PIXEL], outs[var]):
-            assert torch.equal(a, b), (scan, var, (a - b).abs().max().item())
+        for op, a, b in zip(("proj_t", "bproj", "bproj_w"), outs[rt.ADJ_PER_PIXEL], outs[var]):
+            assert torch.equal(a, b), (scan, var, op, (a - b).abs().max().item())
 
 
 def _run_angles(kind):
-    if kind == "full":       # five axis runs, starting inside the first
+    if kind == "full":       # full turn from 0.3 rad
         a = 0.3 + np.arange(90) * (2 * np.pi / 90)
     elif kind == "short":    # half turn plus fan, from a negative angle
         a = -0.2 + np.arange(70) * ((np.pi + 0.6) / 70)
-    elif kind == "shuffled":  # views out of order: more runs than passes
+    elif kind == "shuffled":  # views out of order
         a = np.random.default_rng(5).permutation(np.arange(60) * (2 * np.pi / 60))
     else:                    # one view next to the 45 degree axis switch
         a = np.array([0.7854])
@@ -70,18 +70,18 @@ def _run_angles(kind):
 
 @pytest.mark.parametrize("scan", ["para", "flat", "arc"])
 @pytest.mark.parametrize("size,batch,tile,angles", [
-    ((130, 70), 16, 16, "full"),    # W != H, 130 = 2 * 64 + 2
+    ((130, 70), 16, 16, "full"),    # W != H, one 16-instance tile
     ((130, 70), 16, 8, "full"),     # two instance tiles in one launch
-    ((64, 127), 9, 4, "short"),     # padded tiles; 127 = 2 * 64 - 1
+    ((64, 127), 9, 4, "short"),     # padded tiles
     ((63, 126), 3, 0, "shuffled"),
     ((100, 100), 1, 0, "single"),
 ])
-def test_runs_variant_equals_per_pixel(scan, size, batch, tile, angles):
-    """The axis-run passes (one wavefront per 64 pixels of a line, the last
-    bin of each pixel taken from the next lane) sum the same overlaps in the
-    same order as the per-pixel kernel: equal bit for bit, across line
-    lengths either side of 64, instance tiles, padded batches, and view
-    orders that give more axis runs than passes."""
+def test_adjoint_variants_agree_on_irregular_scans(scan, size, batch, tile, angles):
+    """The per-pixel kernel (each view's search started one view ahead from a
+    guess taken at the pixel centre) and the 2x2-quad and 8x8-tile kernels
+    (guess from the line's edge map, found view by view) sum the same
+    overlaps in the same order: equal bit for bit on rectangular images,
+    instance tiles, padded batches, views out of order and a single view."""
     from pridediff_amd import load_radontorch
     from pridediff_amd.batched import BatchedProjector2d
     from pridediff_amd.deeprecon.torch.projector import projector2d
@@ -92,29 +92,13 @@ def test_runs_variant_equals_per_pixel(scan, size, batch, tile, angles):
     gen = torch.Generator(device="cuda").manual_seed(11)
     y = torch.rand(batch, 1, ang.numel(), 151, device="cuda", generator=gen)
     outs = {}
-    for var in (rt.ADJ_PER_PIXEL, rt.ADJ_RUNS):
+    for var in (rt.ADJ_PER_PIXEL, rt.ADJ_QUAD, rt.ADJ_TILE8):
         BP = BatchedProjector2d(P, ang, batch, tile=tile, variant=var)
         with torch.no_grad():
             outs[var] = (BP.projection_t(y), BP.backprojection(y), BP.backprojection_w(y))
-    for a, b in zip(outs[rt.ADJ_PER_PIXEL], outs[rt.ADJ_RUNS]):
-        assert torch.equal(a, b), (scan, size, angles, (a - b).abs().max().item())
-
-
-def test_runs_variant_in_one_call_ops():
-    """One-call ops (which build their geometry per call) with the axis-run
-    adjoint as the process default equal the per-pixel kernel bit for bit."""
-    from pridediff_amd import load_radontorch
-    rt = load_radontorch()
-    P, ang, x, y = _setup(n=96, d=151, v=90, b=16)
-    with torch.no_grad():
-        want = (P.projection_t(y, ang), P.backprojection(y, ang), P.backprojection_w(y, ang))
-        rt.set_adjoint_variant(rt.ADJ_RUNS)
-        try:
-            got = (P.projection_t(y, ang), P.backprojection(y, ang), P.backprojection_w(y, ang))
-        finally:
-            rt.set_adjoint_variant(-1)
-    for a, b in zip(want, got):
-        assert torch.equal(a, b), (a - b).abs().max().item()
+    for var in (rt.ADJ_QUAD, rt.ADJ_TILE8):
+        for op, a, b in zip(("proj_t", "bproj", "bproj_w"), outs[rt.ADJ_PER_PIXEL], outs[var]):
+            assert torch.equal(a, b), (scan, size, angles, var, op, (a - b).abs().max().item())
 
 
 def test_graph_replay_matches_eager():
