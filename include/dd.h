@@ -152,8 +152,9 @@ dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
  * [batch, H, W], sinogram [batch, views, D]).  dd2d_batch_create is host
  * only; dd2d_batch_setup builds every instance's geometry in the device
  * memory it is given (dd2d_batch_device_bytes) and waits for it once;
- * dd2d_batch_run then runs one op for all instances as two persistent
- * launches (layout, gather), capturable into a HIP graph.  The result for
+ * dd2d_batch_run then runs one op for all instances as two launches
+ * (layout, gather; plus the chunk reduction for image -> sinogram ops),
+ * capturable into a HIP graph.  The result for
  * each instance equals dd2d_run on that instance bit for bit.  No
  * counterpart in the reference, which launches once per instance and op
  * (deeprecon/cuda/dis2d/FanArcDisCUDA2d.cu:758-1016). */
@@ -171,7 +172,8 @@ typedef struct dd2d_batch dd2d_batch;
 
 dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n);
 size_t dd2d_batch_device_bytes(const dd2d_batch* b);
-/* Persistent grid size in workgroups (0 = 8 per compute unit). */
+/* Grid size in workgroups, each striding over the work items (0 = one
+ * workgroup per work item). */
 void dd2d_batch_set_grid(dd2d_batch* b, int workgroups);
 /* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL, DD_ADJ_TILE16 or
  * DD_ADJ_QUAD (in a batch: one thread per 4x4 pixel block, four lines of

@@ -27,7 +27,7 @@ CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
 
 
-KEY_VERSION = 5  # bumped whenever the kernels or the sweep change
+KEY_VERSION = 6  # bumped whenever the kernels or the sweep change
 
 
 def _device_key() -> str:
@@ -110,7 +110,7 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
     return BatchedProjector2d(projector, angles, batch, tile=best[0], variant=best[1])
 
 
-def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(1, 2, 4, 8, 16), variants=None,
+def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(2, 8, 32, 128, 512), variants=None,
                 use_cache: bool = True, report: dict | None = None) -> UnevenBatch2d:
     """Sets the fastest persistent grid size and kernel variant on ``batch``
     (in place)."""
