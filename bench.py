@@ -55,7 +55,7 @@ def parse():
                     help="fanarc: 512^2 fan-arc batch 16 (headline); cone_flat: 256^3 cone-flat "
                          "batch 2; helical: 512x512x64 helical batch 4; fidelity: PrideDiff's "
                          "data-consistency step x - w FBP(A x - y) on the fan-arc batch; "
-                         "uneven: 64 fan-arc instances of uneven size in one persistent launch")
+                         "uneven: 64 fan-arc instances of uneven size in one launch per op")
     return ap.parse_args()
 
 
@@ -89,7 +89,7 @@ def timed(torch, fn, steps, warmup):
 
 def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
     """64 fan-arc instances, N in {256,384,512}, V in [96,720], D = ceil(1.44 N):
-    one persistent launch per op inside one HIP graph vs 64 plain calls."""
+    one launch per op and phase inside one HIP graph vs 64 plain calls."""
     from pridediff_amd.autotune import tune_uneven
     from pridediff_amd.batched import GraphStep, UnevenBatch2d, data_consistency_grad
     from pridediff_amd.deeprecon.torch.projector import projector2d
@@ -411,7 +411,7 @@ def main():
             "vs_baseline": None, "dtype": "f32",
             "data": "synthetic (seeded ellipse phantoms)",
             "config": {"workload": "64 fan-arc instances, N in {256,384,512}, V in [96,720], "
-                                   "D = ceil(1.44 N), one persistent launch per op"},
+                                   "D = ceil(1.44 N), one launch per op"},
             "roofline": {"bound": "hbm", "achieved": c2["algorithmic_GB"] / c2["batched_ms"] * 1e3,
                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": c2["roofline_frac"],
                          "traffic": None, "kernel": "step"},

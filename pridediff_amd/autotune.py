@@ -4,7 +4,7 @@ Sweeps the launch parameters of a planned operator on the problem it will
 run — the instance tile (how many instances share one geometry evaluation,
 1-16) and the sinogram -> image kernel variant (per pixel, 16x16 or 8x8
 tiles with LDS edge positions, or one thread per 2x2 pixels) of a uniform
-batch, the persistent grid size and
+batch, the grid size and
 kernel variant of an uneven batch —
 times one forward projection plus one adjoint per candidate with HIP events,
 keeps the fastest, and remembers it in a JSON cache keyed by the device and
@@ -112,7 +112,7 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
 
 def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(2, 8, 32, 128, 512), variants=None,
                 use_cache: bool = True, report: dict | None = None) -> UnevenBatch2d:
-    """Sets the fastest persistent grid size and kernel variant on ``batch``
+    """Sets the fastest grid size and kernel variant on ``batch``
     (in place)."""
     if variants is None:
         variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD)

@@ -80,7 +80,7 @@ class BatchedProjector2d(torch.nn.Module):
 
 class UnevenBatch2d(torch.nn.Module):
     """Instances of different image size, detector and view count, all run
-    by one persistent launch per op (``radontorch.PlanBatch2d``).
+    by one launch per op and phase (``radontorch.PlanBatch2d``).
 
     Images and sinograms travel packed into flat buffers (``pack_images`` /
     ``unpack_sinos`` and friends); the ops are differentiable.

@@ -413,8 +413,8 @@ class Plan2d {
 
 // ---- batched uneven instances ----------------------------------------------
 // Instances of different image size, detector and view count, packed into
-// one flat image buffer and one flat sinogram buffer; every op is two
-// persistent launches for the whole batch (ddbatch.hip).
+// one flat image buffer and one flat sinogram buffer; every op is two or
+// three launches for the whole batch (ddbatch.hip).
 class PlanBatch2d {
  public:
   PlanBatch2d(int scan, const std::vector<torch::Tensor>& options,
