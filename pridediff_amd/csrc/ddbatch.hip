@@ -81,11 +81,14 @@ constexpr int kBT = 1;  // one geometry per instance
 
 // Block items of the DD_ADJ_QUAD variant: one thread per kBlockQ x kBlockQ
 // pixel block (one first-bin search and one bin walk per line serve kBlockQ
-// pixels), 16 x 16 threads.  Measured on 64 uneven instances: adjoint 17.4 /
-// 16.4 / 14.7 / 17.6 / 21.0 ms for blocks of 2 / 3 / 4 / 5 / 6 (register
-// pressure past 4).
+// pixels), kQuadSide x kQuadSide threads per item.  Measured on 64 uneven
+// instances: adjoint 17.4 / 16.4 / 14.7 / 17.6 / 21.0 ms for blocks of 2 / 3
+// / 4 / 5 / 6 (register pressure past 4); items of 8 x 8 threads (32 x 32
+// pixels) instead of 16 x 16: 13.9 vs 13.6 ms.
 constexpr int kBlockQ = 4;
-constexpr int kQuadTile = 16 * kBlockQ;  // pixels per quad item edge
+constexpr int kQuadSide = 16;
+constexpr int kQuadThreads = kQuadSide * kQuadSide;
+constexpr int kQuadTile = kQuadSide * kBlockQ;  // pixels per quad item edge
 
 __device__ __forceinline__ int find_inst(const char* __restrict__ mem, int n, int phase,
                                          int item) {
@@ -259,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_pixels(
 
 // Block items: (instance, 64 x 64 pixel tile), one thread per 4 x 4 pixels.
 template <int WM>
-__global__ __launch_bounds__(kThreads) void k_batch_quads(
+__global__ __launch_bounds__(kQuadThreads) void k_batch_quads(
     char* __restrict__ mem, int n_inst, int total, float* __restrict__ out) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int i = find_inst(mem, n_inst, kGatherQuads, item);
@@ -269,8 +272,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_quads(
     const int ty = (d.g.H + kQuadTile - 1) / kQuadTile;
     const int t = local % (tx * ty);
     const int b0 = (local / (tx * ty)) * kBT;
-    const int c0 = (t % tx) * kQuadTile + kBlockQ * (threadIdx.x % kTile);
-    const int r0 = (t / tx) * kQuadTile + kBlockQ * (threadIdx.x / kTile);
+    const int c0 = (t % tx) * kQuadTile + kBlockQ * (threadIdx.x % kQuadSide);
+    const int r0 = (t / tx) * kQuadTile + kBlockQ * (threadIdx.x / kQuadSide);
     if (c0 < d.g.W && r0 < d.g.H)
       blocks_body<kBT, WM, kBlockQ>(d.g, d.views, d.edges, d.lines, d.pint, d.sino,
                                     out + d.img_off, r0, c0, b0);
@@ -477,13 +480,13 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
     if (b->variant == DD_ADJ_QUAD) {
       const int qt = b->total[kGatherQuads];
       if (op == DD_PROJ_T)
-        hipLaunchKernelGGL((k_batch_quads<W_FP>), dim3(grid_for(b, qt)), dim3(kThreads), 0, st,
+        hipLaunchKernelGGL((k_batch_quads<W_FP>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0, st,
                            inst, n, qt, output);
       else if (op == DD_BPROJ)
-        hipLaunchKernelGGL((k_batch_quads<W_BP>), dim3(grid_for(b, qt)), dim3(kThreads), 0, st,
+        hipLaunchKernelGGL((k_batch_quads<W_BP>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0, st,
                            inst, n, qt, output);
       else
-        hipLaunchKernelGGL((k_batch_quads<W_BPW>), dim3(grid_for(b, qt)), dim3(kThreads), 0, st,
+        hipLaunchKernelGGL((k_batch_quads<W_BPW>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0, st,
                            inst, n, qt, output);
       return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
     }
