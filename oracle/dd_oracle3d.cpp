@@ -15,9 +15,18 @@
 //
 // Pinned by tests/test_oracle_3d.py: reductions to the pinned 2D oracle
 // (one detector row and one slice that covers it), exact adjointness of each
-// op pair, and spiral with zero source tables equal to cone-arc with a
-// centred volume.  The CUDA kernels' own FP32 behaviour is not reproduced
-// (parity unpinned at the bit level; the GPU is held to this FP64 oracle).
+// op pair, spiral with zero source tables equal to cone-arc with a centred
+// volume, and one exact identity per free shift parameter, each shown to fail
+// on a planted sign slip (ddo_plant3d):
+//   PixZShift = dz           <=> the volume moved down one slice (:103-104);
+//   BinRowShift = row pitch  <=> detector rows moved by one (:68);
+//   SourceShiftRadius = c    <=> cone-arc with SID+c, SDD+c, column pitch and
+//                                shift x SDD/(SDD+c) (SpiralDisCUDA3d.cu:72-75);
+//   SourceShiftAngle = d     <=> cone-arc at view angle + d with column shift
+//                                - d (SDD-SID)/SDD - d (:75-78);
+//   SourceShiftAxial = s     <=> axial position + s with BinRowShift - s (:81-82).
+// The CUDA kernels' own FP32 behaviour is not reproduced (parity unpinned at
+// the bit level; the GPU is held to this FP64 oracle).
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -27,6 +36,20 @@ namespace {
 enum { kConeFlat = 3, kConeArc = 4, kSpiral = 5 };
 enum { kProj = 0, kProjTrans = 1, kBackProj = 2, kBackProjTrans = 3,
        kBackProjW = 4, kBackProjTransW = 5 };
+
+// Planted mistakes for the pinning tests (tests/test_oracle_3d.py): each bit
+// flips one sign or drops one factor, so a test can show that its identity
+// fails on that mistake.  Zero in every normal use.
+enum {
+  kPlantPixZShift = 1,      // z0 - shz instead of z0 + shz
+  kPlantBinRowShift = 2,    // row edges - rsh instead of + rsh
+  kPlantRadius = 4,         // SID - dr, SDD - dr
+  kPlantAngleSource = 8,    // source rotated by -da
+  kPlantAngleColumns = 16,  // column shift + da (SDD'-SID')/SDD'
+  kPlantAxial = 32,         // source z = axial - shift
+  kPlantColPitch = 64,      // column pitch not rescaled by SDD/SDD'
+};
+int g_plant = 0;
 
 struct P3 {
   int scan, B, W, H, Dz, C, R, V;
@@ -47,12 +70,15 @@ void run3d(const P3& g, int op, const double* angles, const double* in,
     if (g.scan == kSpiral) {
       const double pa = g.spos[v], dr = g.spos[g.V + v], da = g.spos[2 * g.V + v],
                    dzs = g.spos[3 * g.V + v];
-      sid = g.sid + dr;
-      sdd = g.sdd + dr;
-      colp = g.col * g.sdd / sdd;
-      csh = g.csh * g.sdd / sdd - da * (sdd - sid) / sdd;
-      th_src_f = static_cast<float>(angles[v]) + static_cast<float>(da);
-      sz = pa + dzs;
+      const double rs = (g_plant & kPlantRadius) ? -dr : dr;
+      sid = g.sid + rs;
+      sdd = g.sdd + rs;
+      colp = (g_plant & kPlantColPitch) ? g.col : g.col * g.sdd / sdd;
+      csh = g.csh * g.sdd / sdd +
+            ((g_plant & kPlantAngleColumns) ? da : -da) * (sdd - sid) / sdd;
+      th_src_f = static_cast<float>(angles[v]) +
+                 static_cast<float>((g_plant & kPlantAngleSource) ? -da : da);
+      sz = pa + ((g_plant & kPlantAxial) ? -dzs : dzs);
       zoff = pa;
     }
     const double th_src = static_cast<double>(th_src_f);
@@ -92,7 +118,7 @@ void run3d(const P3& g, int op, const double* angles, const double* in,
     }
     // detector row edges (z on the detector, or on the virtual detector)
     for (int r = 0; r <= R; ++r) {
-      const double z = (r - R / 2.0) * g.row + g.rsh;
+      const double z = (r - R / 2.0) * g.row + ((g_plant & kPlantBinRowShift) ? -g.rsh : g.rsh);
       zdet[r] = (g.scan == kConeFlat) ? z * g.sid / g.sdd : z + zoff;
     }
     auto zrow = [&](int k, int r) { return mag[k] * (zdet[r] - sz) + sz; };
@@ -107,7 +133,8 @@ void run3d(const P3& g, int op, const double* angles, const double* in,
     };
     const int nline = xaxis ? g.H : g.W;
     const int npix = xaxis ? g.W : g.H;
-    const double z0v = -g.Dz / 2.0 * g.dz + (g.scan == kSpiral ? 0.0 : g.shz);
+    const double shz = (g_plant & kPlantPixZShift) ? -g.shz : g.shz;
+    const double z0v = -g.Dz / 2.0 * g.dz + (g.scan == kSpiral ? 0.0 : shz);
     for (int l = 0; l < nline; ++l) {
       double a0, p, m;
       if (xaxis) {
@@ -187,6 +214,9 @@ void run3d(const P3& g, int op, const double* angles, const double* in,
 }  // namespace
 
 extern "C" {
+
+// Sets the planted-mistake mask (test infrastructure; 0 restores the oracle).
+void ddo_plant3d(int mask) { g_plant = mask; }
 
 // 3D op in FP64.  scan: 3 cone-flat, 4 cone-arc, 5 spiral.  `image` is
 // [B, Dz, H, W], `proj` [B, V, rows, cols]; the output side is accumulated

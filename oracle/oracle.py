@@ -13,6 +13,7 @@ Geometry parameters follow the reference's 2D ``Options`` layout
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 from dataclasses import dataclass
@@ -182,6 +183,25 @@ def _bind3d(lib):
     i, d = ctypes.c_int, ctypes.c_double
     lib.ddo_op3d_f64.argtypes = [i, i, _F64P, _F64P, _F64P, _F64P] + [i] * 7 + [d] * 12
     lib.ddo_op3d_f64.restype = i
+    lib.ddo_plant3d.argtypes = [i]
+    lib.ddo_plant3d.restype = None
+
+
+# planted-mistake bits of dd_oracle3d.cpp (tests only)
+PLANT3D = {"pix_z_shift": 1, "bin_row_shift": 2, "radius": 4, "angle_source": 8,
+           "angle_columns": 16, "axial": 32, "col_pitch": 64}
+
+
+@contextlib.contextmanager
+def planted3d(name: str):
+    """Runs the 3D oracle with one planted mistake (a flipped sign or a
+    dropped factor), to show that a pinning identity fails on it."""
+    lib = _load_oracle()
+    lib.ddo_plant3d(PLANT3D[name])
+    try:
+        yield
+    finally:
+        lib.ddo_plant3d(0)
 
 
 @dataclass(frozen=True)

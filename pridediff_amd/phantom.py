@@ -47,19 +47,32 @@ def ellipsoids3d(nx: int, ny: int, nz: int, batch: int = 1, count: int = 20,
     xs = (np.arange(nx) + 0.5) / nx * 2.0 - 1.0
     ys = 1.0 - (np.arange(ny) + 0.5) / ny * 2.0
     zs = (np.arange(nz) + 0.5) / nz * 2.0 - 1.0
-    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+
+    def window(coords, lo, hi):
+        # indices whose coordinate lies in [lo, hi] (coords monotone), one voxel of margin
+        idx = np.nonzero((coords >= lo) & (coords <= hi))[0]
+        if idx.size == 0:
+            return slice(0, 0)
+        return slice(max(int(idx[0]) - 1, 0), min(int(idx[-1]) + 2, coords.size))
+
     out = np.zeros((batch, nz, ny, nx), dtype=np.float32)
     for b in range(batch):
-        acc = np.zeros((nz, ny, nx), dtype=np.float32)
+        acc = out[b]
         for _ in range(count):
             c = rng.uniform(-0.5, 0.5, size=3)
             ax = rng.uniform(0.05, 0.4, size=3)
             t = rng.uniform(0.0, np.pi)
             val = np.float32(rng.uniform(0.1, 1.0))
             ct, st = np.cos(t), np.sin(t)
+            # the ellipsoid lies inside the box |x - cx|, |y - cy| <= max(a, b),
+            # |z - cz| <= c; the inclusion test is evaluated only there
+            rxy = max(ax[0], ax[1])
+            sx = window(xs, c[0] - rxy, c[0] + rxy)
+            sy = window(ys, c[1] - rxy, c[1] + rxy)
+            sz = window(zs, c[2] - ax[2], c[2] + ax[2])
+            Z, Y, X = np.meshgrid(zs[sz], ys[sy], xs[sx], indexing="ij")
             u = (X - c[0]) * ct + (Y - c[1]) * st
             v = -(X - c[0]) * st + (Y - c[1]) * ct
             w = Z - c[2]
-            acc[(u / ax[0]) ** 2 + (v / ax[1]) ** 2 + (w / ax[2]) ** 2 <= 1.0] += val
-        out[b] = acc
+            acc[sz, sy, sx][(u / ax[0]) ** 2 + (v / ax[1]) ** 2 + (w / ax[2]) ** 2 <= 1.0] += val
     return out

@@ -129,3 +129,87 @@ def test_cone_reduces_to_fan_on_gpu(rt):
     p2 = fn(torch.from_numpy(x[:, 0])[:, None].cuda(), torch.tensor(g2.options(), dtype=torch.float32),
             torch.from_numpy(ang).cuda())[:, 0].cpu().numpy()
     assert report("cone-arc 1-row vs fan-arc (GPU)", p3, p2) <= 1e-5
+
+
+# ---- the shift-parameter identities of the FP64 oracle, on the GPU ---------
+def _g3(scan, **kw):
+    base = dict(W=32, H=30, Dz=20, C=48, R=12, dx=1.0, dy=1.125, dz=1.25,
+                col=1.5 if scan == "cone_flat" else 2.0 ** -6, row=1.5, sid=96.0, sdd=192.0,
+                shx=0.25, shy=-0.375, shz=0.0,
+                csh=0.25 if scan == "cone_flat" else 2.0 ** -7, rsh=0.0)
+    base.update(kw)
+    return O.Geom3D(**base)
+
+
+def _ident(tag, a, b):
+    r = report(f"identity {tag}", a, b)
+    assert r <= 1e-6
+    return r
+
+
+def _views(v):
+    return (np.arange(v, dtype=np.float64) * 0.125 + 0.0625).astype(np.float32)
+
+
+@pytest.mark.parametrize("scan", ["cone_flat", "cone_arc"])
+def test_gpu_pix_z_shift_and_bin_row_shift_identities(rt, scan):
+    g0 = _g3(scan)
+    ang = _views(40)
+    rng = np.random.default_rng(70)
+    x = rng.uniform(0, 1, (2, g0.Dz, g0.H, g0.W)).astype(np.float32)
+    x[:, 0] = 0.0
+    y = rng.uniform(0, 1, (2, 40, g0.R, g0.C)).astype(np.float32)
+    gz = _g3(scan, shz=g0.dz)
+    for op in ("proj", "bproj_tw"):
+        got = gpu_op3(rt, scan, op, np.roll(x, -1, axis=1), ang, gz)
+        _ident(f"{scan} PixZShift {op}", got, gpu_op3(rt, scan, op, x, ang, g0))
+        assert report(f"{scan} PixZShift {op} vs f64", got,
+                      O.op3d(scan, op, np.roll(x, -1, axis=1), ang, gz)) <= 1e-5
+    for op in ("proj_t", "bproj"):
+        _ident(f"{scan} PixZShift {op}", gpu_op3(rt, scan, op, y, ang, gz)[:, :-1],
+               gpu_op3(rt, scan, op, y, ang, g0)[:, 1:])
+    gr = _g3(scan, rsh=g0.row)
+    yr = y.copy()
+    yr[:, :, 0] = 0.0
+    _ident(f"{scan} BinRowShift proj", gpu_op3(rt, scan, "proj", x, ang, gr)[:, :, :-1],
+           gpu_op3(rt, scan, "proj", x, ang, g0)[:, :, 1:])
+    _ident(f"{scan} BinRowShift proj_t", gpu_op3(rt, scan, "proj_t", np.roll(yr, -1, axis=2), ang, gr),
+           gpu_op3(rt, scan, "proj_t", yr, ang, g0))
+
+
+def test_gpu_spiral_source_shift_identities(rt):
+    gs = _g3("spiral")
+    v = 40
+    ang = _views(v)
+    rng = np.random.default_rng(71)
+    x = rng.uniform(0, 1, (1, gs.Dz, gs.H, gs.W)).astype(np.float32)
+    y = rng.uniform(0, 1, (1, v, gs.R, gs.C)).astype(np.float32)
+    zero = np.zeros((4, v), dtype=np.float32)
+    # radius shift c: cone-arc with SID + c, SDD + c, column pitch/shift x SDD/(SDD + c)
+    c = 64.0
+    sp = zero.copy()
+    sp[1] = c
+    ga = _g3("cone_arc", sid=gs.sid + c, sdd=gs.sdd + c, col=gs.col * 0.75, csh=gs.csh * 0.75)
+    # angle shift d: cone-arc at angle + d, column shift - d (SDD - SID)/SDD - d
+    d = 2.0 ** -6
+    sa = zero.copy()
+    sa[2] = d
+    gd = _g3("cone_arc", csh=gs.csh - 1.5 * d)
+    ang_d = (ang + np.float32(d)).astype(np.float32)
+    # axial focal-spot shift s: axial + s with BinRowShift - s
+    s, r = 0.25, 0.5
+    axial = (np.arange(v) * 0.25 - 5.0).astype(np.float32)
+    s1 = np.stack([axial, zero[0], zero[0], np.full(v, s, np.float32)])
+    s2 = np.stack([axial + np.float32(s), zero[0], zero[0], zero[0]])
+    g1, g2 = _g3("spiral", rsh=r), _g3("spiral", rsh=r - s)
+    for op in ("proj", "proj_t", "bproj_w"):
+        inp = x if op in O.TO_SINO else y
+        _ident(f"SourceShiftRadius {op}", gpu_op3(rt, "spiral", op, inp, ang, gs, sp),
+               gpu_op3(rt, "cone_arc", op, inp, ang, ga))
+        _ident(f"SourceShiftAngle {op}", gpu_op3(rt, "spiral", op, inp, ang, gs, sa),
+               gpu_op3(rt, "cone_arc", op, inp, ang_d, gd))
+        _ident(f"SourceShiftAxial {op}", gpu_op3(rt, "spiral", op, inp, ang, g1, s1),
+               gpu_op3(rt, "spiral", op, inp, ang, g2, s2))
+        assert report(f"spiral radius+angle+axial shifts {op} vs f64",
+                      gpu_op3(rt, "spiral", op, inp, ang, g1, s1 + sp + sa),
+                      O.op3d("spiral", op, inp, ang, g1, s1 + sp + sa)) <= 1e-5

@@ -11,7 +11,11 @@ mathematics fixes:
   with cos3D the reference's TriAngCos of the cell centre (:21-23, :100);
 * exact adjointness of every op pair;
 * spiral with zero source tables equals cone-arc with zero z shift
-  (SpiralDisCUDA3d.cu:72-83 vs ConeArcDisCUDA3d.cu).
+  (SpiralDisCUDA3d.cu:72-83 vs ConeArcDisCUDA3d.cu);
+* one exact identity per free shift parameter (PixZShift, BinRowShift, the
+  spiral's SourceShiftRadius / SourceShiftAngle / SourceShiftAxial), each
+  also run against the oracle with a planted sign slip or dropped factor
+  (oracle.planted3d), which it must fail.
 """
 import numpy as np
 import pytest
@@ -127,3 +131,154 @@ def test_spiral_axial_shift_moves_the_projection():
     a = O.op3d("spiral", "proj", shifted, ang, g, spos)
     b = O.op3d("spiral", "proj", x, ang, g, zero)
     assert 1e-7 < O.rel_l2(a, b) < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# Exact identities for the free shift parameters.  All geometry values are
+# dyadic (exact in float32, and in every sum and product the identities
+# rearrange), so the two sides differ only by FP64 rounding; each identity is
+# also shown to fail when the oracle carries a planted sign slip.
+
+def _dyadic_views(v, step=0.125, off=0.0625):
+    return (np.arange(v, dtype=np.float64) * step + off).astype(np.float32)
+
+
+def _g3(scan, **kw):
+    sdd = 192.0
+    base = dict(W=14, H=12, Dz=10, C=25, R=7, dx=1.0, dy=1.125, dz=1.25,
+                col=1.5 if scan == "cone_flat" else 2.0 ** -6, row=1.5, sid=96.0, sdd=sdd,
+                shx=0.25, shy=-0.375, shz=0.0,
+                csh=0.25 if scan == "cone_flat" else 2.0 ** -7, rsh=0.0)
+    base.update(kw)
+    return O.Geom3D(**base)
+
+
+def _rand(shape, seed):
+    return np.random.default_rng(seed).uniform(0.0, 1.0, shape)
+
+
+def _assert_identity(check, plant):
+    """check() returns the relative L2 mismatch of the identity."""
+    assert check() < 1e-12
+    with O.planted3d(plant):
+        assert check() > 1e-6, plant
+
+
+@pytest.mark.parametrize("scan", ["cone_flat", "cone_arc"])
+def test_pix_z_shift_moves_the_volume_one_slice(scan):
+    """PixZShift = dz puts slice s where slice s+1 was
+    (ConeFlatDisCUDA3d.cu:103-104, ConeArcDisCUDA3d.cu:105-106): projecting
+    the volume moved down one slice with the shift equals projecting it
+    unshifted, and the backprojections are the same volumes moved by one."""
+    g0 = _g3(scan)
+    g1 = _g3(scan, shz=g0.dz)
+    ang = _dyadic_views(16)
+    x = _rand((2, g0.Dz, g0.H, g0.W), 60)
+    x[:, 0] = 0.0                                   # nothing leaves the grid
+    x_down = np.roll(x, -1, axis=1)
+    y = _rand((2, 16, g0.R, g0.C), 61)
+
+    def check():
+        errs = []
+        for op in ("proj", "bproj_t", "bproj_tw"):
+            errs.append(O.rel_l2(O.op3d(scan, op, x_down, ang, g1), O.op3d(scan, op, x, ang, g0)))
+        for op in ("proj_t", "bproj", "bproj_w"):
+            a = O.op3d(scan, op, y, ang, g1)[:, :-1]
+            b = O.op3d(scan, op, y, ang, g0)[:, 1:]
+            errs.append(O.rel_l2(a, b))
+        return max(errs)
+
+    _assert_identity(check, "pix_z_shift")
+
+
+@pytest.mark.parametrize("scan", ["cone_flat", "cone_arc", "spiral"])
+def test_bin_row_shift_moves_the_rows_by_one(scan):
+    """BinRowShift = one row pitch makes row r the old row r+1
+    (ConeFlatDisCUDA3d.cu:68, ConeArcDisCUDA3d.cu:73, SpiralDisCUDA3d.cu:82)."""
+    g0 = _g3(scan)
+    g1 = _g3(scan, rsh=g0.row)
+    ang = _dyadic_views(16)
+    spos = None
+    if scan == "spiral":
+        spos = np.stack([np.arange(16) * 0.25 - 2.0, np.zeros(16), np.zeros(16),
+                         np.full(16, 0.5)]).astype(np.float32)
+    x = _rand((2, g0.Dz, g0.H, g0.W), 62)
+    y = _rand((2, 16, g0.R, g0.C), 63)
+    y[:, :, 0] = 0.0
+    y_down = np.roll(y, -1, axis=2)
+
+    def check():
+        errs = []
+        for op in ("proj", "bproj_t", "bproj_tw"):
+            a = O.op3d(scan, op, x, ang, g1, spos)[:, :, :-1]
+            b = O.op3d(scan, op, x, ang, g0, spos)[:, :, 1:]
+            errs.append(O.rel_l2(a, b))
+        for op in ("proj_t", "bproj", "bproj_w"):
+            errs.append(O.rel_l2(O.op3d(scan, op, y_down, ang, g1, spos),
+                                 O.op3d(scan, op, y, ang, g0, spos)))
+        return max(errs)
+
+    _assert_identity(check, "bin_row_shift")
+
+
+ALL_OPS = ("proj", "proj_t", "bproj", "bproj_t", "bproj_w", "bproj_tw")
+
+
+def _all_ops_rel(run_a, run_b, x, y):
+    return max(O.rel_l2(run_a(op, x if op in O.TO_SINO else y),
+                        run_b(op, x if op in O.TO_SINO else y)) for op in ALL_OPS)
+
+
+@pytest.mark.parametrize("plant", ["radius", "col_pitch"])
+def test_source_shift_radius_equals_cone_arc_with_moved_source(plant):
+    """SourceShiftRadius c (SpiralDisCUDA3d.cu:72-75) moves source and detector
+    out by c and keeps the detector's absolute column pitch: cone-arc with
+    SID + c, SDD + c and column pitch and shift scaled by SDD / (SDD + c)."""
+    c = 64.0                                         # SDD / (SDD + c) = 0.75
+    gs = _g3("spiral")
+    ga = _g3("cone_arc", sid=gs.sid + c, sdd=gs.sdd + c, col=gs.col * 0.75, csh=gs.csh * 0.75)
+    ang = _dyadic_views(16)
+    spos = np.zeros((4, 16), dtype=np.float32)
+    spos[1] = c
+    x = _rand((1, gs.Dz, gs.H, gs.W), 64)
+    y = _rand((1, 16, gs.R, gs.C), 65)
+    _assert_identity(lambda: _all_ops_rel(lambda op, t: O.op3d("spiral", op, t, ang, gs, spos),
+                                          lambda op, t: O.op3d("cone_arc", op, t, ang, ga), x, y),
+                     plant)
+
+
+@pytest.mark.parametrize("plant", ["angle_source", "angle_columns"])
+def test_source_shift_angle_equals_cone_arc_at_the_shifted_view(plant):
+    """SourceShiftAngle d (flying focal spot, SpiralDisCUDA3d.cu:75-78)
+    rotates the source by d while the detector stays: cone-arc at view
+    angle + d with column shift - d (SDD - SID) / SDD - d."""
+    d = 2.0 ** -6
+    gs = _g3("spiral")                               # (SDD - SID) / SDD = 0.5
+    ga = _g3("cone_arc", csh=gs.csh - 0.5 * d - d)
+    ang = _dyadic_views(16)
+    spos = np.zeros((4, 16), dtype=np.float32)
+    spos[2] = d
+    ang_a = (ang + np.float32(d)).astype(np.float32)
+    x = _rand((1, gs.Dz, gs.H, gs.W), 66)
+    y = _rand((1, 16, gs.R, gs.C), 67)
+    _assert_identity(lambda: _all_ops_rel(lambda op, t: O.op3d("spiral", op, t, ang, gs, spos),
+                                          lambda op, t: O.op3d("cone_arc", op, t, ang_a, ga), x, y),
+                     plant)
+
+
+def test_source_shift_axial_trades_with_axial_position_and_row_shift():
+    """SourceShiftAxial s moves only the source (SpiralDisCUDA3d.cu:81-82):
+    (axial a, shift s, BinRowShift r) equals (axial a + s, shift 0,
+    BinRowShift r - s)."""
+    s, r = 0.25, 0.5
+    g1 = _g3("spiral", rsh=r)
+    g2 = _g3("spiral", rsh=r - s)
+    ang = _dyadic_views(16)
+    axial = (np.arange(16) * 0.25 - 2.0).astype(np.float32)
+    sp1 = np.stack([axial, np.zeros(16), np.zeros(16), np.full(16, s)]).astype(np.float32)
+    sp2 = np.stack([axial + np.float32(s), np.zeros(16), np.zeros(16), np.zeros(16)]).astype(np.float32)
+    x = _rand((1, g1.Dz, g1.H, g1.W), 68)
+    y = _rand((1, 16, g1.R, g1.C), 69)
+    _assert_identity(lambda: _all_ops_rel(lambda op, t: O.op3d("spiral", op, t, ang, g1, sp1),
+                                          lambda op, t: O.op3d("spiral", op, t, ang, g2, sp2), x, y),
+                     "axial")
