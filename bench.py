@@ -45,6 +45,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 oracle check")
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven-batch measurement")
+    ap.add_argument("--no-3d", action="store_true",
+                    help="skip the cone-flat and helical legs of the default run")
+    ap.add_argument("--no-c0", action="store_true", help="skip the parallel-beam CPU-case leg")
+    ap.add_argument("--no-dc-loop", action="store_true",
+                    help="skip the helical workload's 50-step descent loop (profiling runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner sweep (automatic tile, default kernel variant); "
                          "for profiling runs")
@@ -76,6 +81,56 @@ def workload(torch, dev):
     return P, ang, ang_np, x_true, x0, y.contiguous()
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "kernel_counters.json")
+
+
+def _counters():
+    try:
+        with open(COUNTERS) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def roofline_for(key, kernel, ms, b_alg, step_ms=None, n_ops=2):
+    """Roofline of one gather kernel.
+
+    The gathers are bound by the per-CU vector-memory path (texture
+    addresser -> L1, 64 B per clock per CU), not by HBM: every (pixel, view)
+    value passes through it, while the caches keep most re-reads off HBM.
+    ``achieved`` = the bytes the kernel moves through that path per launch
+    (rocprofv3 counters of the same launch, committed beside this script) /
+    the launch time measured here with HIP events; ``peak`` = CUs x 64 B/clk
+    x the measured shader clock.  HBM is reported as the real fraction of
+    8 TB/s (FETCH_SIZE + WRITE_SIZE per launch / time), and ``effective_GBps``
+    is the algorithmic view-pass bytes / time (it exceeds the HBM peak
+    because caches serve the per-view re-reads).
+    """
+    c = _counters()
+    k = c.get("kernels", {}).get(key, {})
+    t = ms * 1e-3
+    out = {"bound": "vmem", "achieved": None, "peak": None, "unit": "GB/s", "frac": None,
+           "traffic": k.get("hbm_bytes"), "kernel": kernel,
+           "effective_GBps": b_alg / t / 1e9, "effective_frac_of_hbm_peak": b_alg / t / HBM_PEAK}
+    if step_ms:
+        out["step_effective_frac_of_hbm_peak"] = n_ops * b_alg / (step_ms * 1e-3) / HBM_PEAK
+    if k.get("vmem_bytes") and c.get("clock_hz") and c.get("cus"):
+        peak = c["cus"] * c.get("l1_bytes_per_clk", 64) * c["clock_hz"]
+        out.update(achieved=k["vmem_bytes"] / t / 1e9, peak=peak / 1e9,
+                   frac=k["vmem_bytes"] / t / peak)
+    if k.get("hbm_bytes"):
+        out["hbm"] = {"achieved": k["hbm_bytes"] / t / 1e9, "peak": HBM_PEAK / 1e9,
+                      "frac": k["hbm_bytes"] / t / HBM_PEAK}
+    if k:
+        out["counts_from"] = ("rocprofv3 --pmc runs of this workload, "
+                              "profiles/kernel_counters.md; times measured in this run")
+    return out
+
+
+def roofline_c1(kernel, ms, b_alg, step_s):
+    return roofline_for("fanarc_b16:" + kernel, kernel, ms, b_alg, step_s * 1e3)
+
+
 def timed(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -87,7 +142,7 @@ def timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
+def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True, cpu=False):
     """64 fan-arc instances, N in {256,384,512}, V in [96,720], D = ceil(1.44 N):
     one launch per op and phase inside one HIP graph vs 64 plain calls."""
     from pridediff_amd.autotune import tune_uneven
@@ -137,10 +192,16 @@ def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
     b_alg = sum(8.0 * v * (h * w + d) for h, w, v, d in shapes)
     out = {"instances": 64, "batched_ms": t_b * 1e3, "plain_ms": t_p * 1e3,
            "detector_samples": sum(v * d for _, _, v, d in shapes),
-           "speedup": t_p / t_b, "roofline_frac": b_alg / t_b / HBM_PEAK,
+           "speedup": t_p / t_b, "step_effective_frac_of_hbm_peak": b_alg / t_b / HBM_PEAK,
            "algorithmic_GB": b_alg / 1e9, "grid": tune.get("grid"),
            "variant": tune.get("variant"),
            "forward_ms": fp_ms / max(fp_n, 1), "adjoint_ms": bp_ms / max(bp_n, 1)}
+    adj_name = {rt.ADJ_QUAD: "k_batch_quads", rt.ADJ_BLOCK_WINDOW: "k_batch_quads_window",
+                rt.ADJ_TILE16: "k_batch_tile"}.get(tune.get("variant", rt.ADJ_QUAD),
+                                                   "k_batch_pixels")
+    dom = (("k_batch_bins", out["forward_ms"]) if out["forward_ms"] >= out["adjoint_ms"]
+           else (adj_name, out["adjoint_ms"]))
+    out["roofline"] = roofline_for("uneven64:" + dom[0], dom[0], dom[1], b_alg / 2)
     if check:
         from oracle import oracle as O
         j = min(range(64), key=lambda i: shapes[i][0] * shapes[i][2])
@@ -150,10 +211,18 @@ def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True):
             got = ub.unpack_sinos(ub.projection(x))[j].cpu().numpy()[None]
         want = O.op2d("arc", "proj", imgs[j].cpu().numpy()[None], angs_np[j], g, "f64")
         out["smallest_instance_fp_rel_l2_vs_fp64"] = O.rel_l2(got, want)
+    if cpu:
+        # the lightest, a middle and the heaviest instance by V * N^2
+        order = sorted(range(64), key=lambda i: shapes[i][2] * shapes[i][0] * shapes[i][1])
+        pick = [order[0], order[32], order[-1]]
+        out["cpu_sample"] = c2_cpu_sample([(i, shapes[i]) for i in pick])
+        per_inst_gpu = out["batched_ms"] * 1e-3 / 64
+        out["cpu_sample"]["note"] = (f"batched GPU step {per_inst_gpu * 1e3:.3f} ms per instance "
+                                     "on average over the 64")
     return out
 
 
-def bench3d(torch, kind, steps, warmup, check):
+def bench3d(torch, kind, steps, warmup, check, dc_loop=True):
     """3D autograd FP + adjoint step, replayed from a captured HIP graph.
 
     cone_flat: 256^3 volume of 20 ellipsoids (1 mm voxels), flat detector
@@ -206,8 +275,8 @@ def bench3d(torch, kind, steps, warmup, check):
     step = GraphStep(lambda a, b: data_consistency_grad(fwd, a, b), x0.clone().requires_grad_(True), y)
     t = timed(torch, lambda: step.replay(x0, y), steps, warmup)
 
-    dc_loop = None
-    if kind == "helical":
+    loop_out = None
+    if kind == "helical" and dc_loop:
         # 50 descent steps on ||A x - y||^2, each one replay of a captured
         # graph holding A x, the residual, A^T through autograd and the update;
         # fixed step from an exact line search at the start point.
@@ -235,7 +304,7 @@ def bench3d(torch, kind, steps, warmup, check):
         t_loop = time.perf_counter() - t0
         with torch.no_grad():
             losses.append(float(((fwd(xk) - y) ** 2).sum()))
-        dc_loop = {"steps": 50, "seconds": t_loop, "ms_per_step": t_loop / 50 * 1e3,
+        loop_out = {"steps": 50, "seconds": t_loop, "ms_per_step": t_loop / 50 * 1e3,
                    "loss_start": losses[0], "loss_end": losses[1], "step_size": alpha,
                    "start": "zero volume"}
 
@@ -270,17 +339,15 @@ def bench3d(torch, kind, steps, warmup, check):
         "data": "synthetic (seeded ellipsoid phantoms, y = 1.001 A x_true)",
         "config": {"workload": desc, "volume": [W, H, Dz], "views": V, "detector": [C, R],
                    "batch": Bn},
-        "roofline": {"bound": "hbm", "achieved": b_alg / (dom_ms * 1e-3) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": b_alg / (dom_ms * 1e-3) / HBM_PEAK, "traffic": None,
-                     "kernel": dom, "step_frac": 2 * b_alg / t / HBM_PEAK,
-                     "in_cone_voxel_views": in_cone},
-        "kernels": {"k_bins3d (A, volume->sino)": {"ms": fp_ms, "GB/s": b_alg / fp_ms / 1e6},
-                    "k_voxels3d (A^T, sino->volume)": {"ms": bp_ms, "GB/s": b_alg / bp_ms / 1e6}},
+        "roofline": dict(roofline_for(f"{kind}:{dom}", dom, dom_ms, b_alg, t * 1e3),
+                         in_cone_voxel_views=in_cone),
+        "kernels": {"k_bins3d (A, volume->sino)": {"ms": fp_ms, "effective_GBps": b_alg / fp_ms / 1e6},
+                    "k_voxels3d (A^T, sino->volume)": {"ms": bp_ms,
+                                                       "effective_GBps": b_alg / bp_ms / 1e6}},
         "cpu_baseline": None,
     }
-    if dc_loop is not None:
-        out["dc_loop"] = dc_loop
+    if loop_out is not None:
+        out["dc_loop"] = loop_out
     if check:
         from oracle import oracle as O
         if kind == "cone_flat":
@@ -289,34 +356,59 @@ def bench3d(torch, kind, steps, warmup, check):
         else:
             g = O.Geom3D(W, H, Dz, C, R, 0.7, 0.7, 1.0, 1.2858 / 1085.6, 1.82454, 595.0, 1085.6)
             scan = "spiral"
-        sel = np.array([V // 2, V // 2 + 7])
+        # full-size launches, checked on sampled views: the forward's views
+        # against the oracle on those views, the adjoint of a sinogram that
+        # is zero outside them against the oracle's adjoint of those views
+        sel = np.linspace(0, V - 1, 4).round().astype(np.int64)
         a_sub = ang_np[sel]
         s_sub = None if spos_np is None else np.ascontiguousarray(spos_np[:, sel])
+        sp_args = () if spos is None else (spos,)
         with torch.no_grad():
-            a_t = torch.from_numpy(a_sub).to(dev)
-            got = (P.projection(x_true[:1], a_t) if s_sub is None else
-                   P.projection(x_true[:1], a_t, torch.from_numpy(s_sub).to(dev)))
+            got = P.projection(x_true[:1], ang, *sp_args)[:, 0, sel].cpu().numpy()
         want = O.op3d(scan, "proj", x_true[:1, 0].cpu().numpy(), a_sub, g, s_sub)
-        ysub = torch.rand(1, 1, len(sel), R, C, device=dev, generator=torch.Generator(
-            device=dev).manual_seed(11))
-        with torch.no_grad():
-            got_t = (P.projection_t(ysub, a_t) if s_sub is None else
-                     P.projection_t(ysub, a_t, torch.from_numpy(s_sub).to(dev)))
-        want_t = O.op3d(scan, "proj_t", ysub[:, 0].cpu().numpy(), a_sub, g, s_sub)
-        out["accuracy"] = {"fp_rel_l2_vs_fp64_2views": O.rel_l2(got[:, 0].cpu().numpy(), want),
-                           "adjoint_rel_l2_vs_fp64_2views": O.rel_l2(got_t[:, 0].cpu().numpy(),
-                                                                     want_t)}
+        # the batch of the workload (so the launch uses its instance tile);
+        # instance 0 is compared
+        ysp = torch.zeros(Bn, 1, V, R, C, device=dev)
+        ysp[:, :, torch.from_numpy(sel).to(dev)] = torch.rand(
+            Bn, 1, len(sel), R, C, device=dev, generator=torch.Generator(device=dev).manual_seed(11))
+        want_t = O.op3d(scan, "proj_t", ysp[:1, 0, torch.from_numpy(sel).to(dev)].cpu().numpy(),
+                        a_sub, g, s_sub)
+        acc = {"views_checked": sel.tolist(),
+               "fp_rel_l2_vs_fp64": O.rel_l2(got, want), "fp_max_abs_vs_fp64": O.max_abs(got, want)}
+        # the adjoint with compensated running sums (the default) and with
+        # running sums rounded to FP32, accuracy and kernel time of each
+        for comp in (True, False):
+            rt.set_compensated3d(1 if comp else 0)
+            with torch.no_grad():
+                got_t = P.projection_t(ysp, ang, *sp_args)[:1, 0].cpu().numpy()
+                rt.profile_enable(True)
+                rt.profile_read(1)
+                for _ in range(2):
+                    P.projection_t(y, ang, *sp_args)
+                torch.cuda.synchronize()
+                ms, n = rt.profile_read(1)
+                rt.profile_enable(False)
+            tag = "compensated" if comp else "rounded"
+            acc[f"adjoint_{tag}"] = {"rel_l2_vs_fp64": O.rel_l2(got_t, want_t),
+                                     "max_abs_vs_fp64": O.max_abs(got_t, want_t),
+                                     "k_voxels3d_ms": ms / max(n, 1)}
+        rt.set_compensated3d(2)
+        acc["default_mode"] = ("compensated" if Bn <= 2 else "rounded") + \
+            " (automatic: compensated at instance tiles 1-2)"
+        out["accuracy"] = acc
     return out
 
 
 def bench_fidelity(torch, steps, warmup, check):
     """PrideDiff's data-consistency step x - w * FBP(A x - y)
-    (model/pridediff.py:56-68) on the 512^2 fan-arc batch of 16:
-    planned path in one HIP graph vs the drop-in module called on the batch
-    (GEMM filter) vs the same with the reference's conv2d filter."""
+    (model/pridediff.py:56-68) on the 512^2 fan-arc batch of 16: the fused
+    planned step (residual and update in the projector kernels' writes) in
+    one HIP graph, vs the unfused planned step in a graph, vs the drop-in
+    module called on the batch (GEMM filter), vs the same with the
+    reference's conv2d filter."""
     from pridediff_amd.autotune import tune_uniform
     from pridediff_amd.batched import GraphStep
-    from pridediff_amd.fbp import PlannedFBP2d, fidelity_step
+    from pridediff_amd.fbp import PlannedFBP2d, fidelity_step, fidelity_step_fused
 
     dev = torch.device("cuda:0")
     P, ang, ang_np, x_true, x0, y = workload(torch, dev)
@@ -324,9 +416,13 @@ def bench_fidelity(torch, steps, warmup, check):
     fbp = PlannedFBP2d(P, ang, B, plan=BP)
     w = torch.tensor(0.5, device=dev)
     with torch.no_grad():
+        # fused: residual in the forward's reduction, update in the BP's write
+        fused = GraphStep(lambda a, c: fidelity_step_fused(fbp, a, c, w.reshape(1)), x0, y)
+        t_graph = timed(torch, lambda: fused.replay(x0, y), steps, warmup)
+        out_graph = fused.replay(x0, y).clone()
         step = GraphStep(lambda a, c: fidelity_step(BP.projection, fbp, a, c, w), x0, y)
-        t_graph = timed(torch, lambda: step.replay(x0, y), steps, warmup)
-        out_graph = step.replay(x0, y).clone()
+        t_unfused = timed(torch, lambda: step.replay(x0, y), steps, warmup)
+        out_unfused = step.replay(x0, y).clone()
         t_module = timed(torch, lambda: x0 - w * P.filtered_backprojection(
             P.projection(x0, ang) - y, ang), max(3, steps // 4), 1)
         out_module = x0 - w * P.filtered_backprojection(P.projection(x0, ang) - y, ang)
@@ -351,10 +447,13 @@ def bench_fidelity(torch, steps, warmup, check):
         "data": "synthetic (seeded ellipse phantoms, Poisson noise at 1e5 photons)",
         "config": {"workload": "x - w FBP(A x - y), fan-arc 512x512, 720 views, 736 bins, batch 16",
                    "image": [N, N], "views": V, "bins": D, "batch": B},
+        "unfused_graph_ms": t_unfused * 1e3,
         "module_call_ms": t_module * 1e3,
         "module_call_conv2d_filter_ms": t_conv * 1e3,
         "filter_stage_ms": {"gemm": t_gemm_only * 1e3, "conv2d": t_conv_only * 1e3},
         "agreement": {
+            "fused_vs_unfused_rel_l2": float((out_graph - out_unfused).double().norm()
+                                             / out_unfused.double().norm()),
             "graph_vs_module_rel_l2": float((out_graph - out_module).double().norm()
                                             / out_module.double().norm()),
             "graph_vs_conv2d_rel_l2": float((out_graph - out_conv).double().norm()
@@ -364,6 +463,23 @@ def bench_fidelity(torch, steps, warmup, check):
     return out
 
 
+def _ref_threads(views, mode):
+    """deeprecon's CPU thread policy min(ceil(V/128), hardware_concurrency)
+    (FanArcDisCPU2d.cpp:820-822); the FP32 port is single-threaded."""
+    hw = os.cpu_count() or 1
+    return (min(math.ceil(views / 128), hw) if mode == "ref" else 1), hw
+
+
+def _cpu_time(fn, runs=3):
+    fn()  # warm-up
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
 def cpu_baseline(ang_np, x_np, y_np):
     """Reference CPU code (oracle/_ref) on 2 slices: FP + ProjTrans, median of 3."""
     from oracle import oracle as O
@@ -371,24 +487,101 @@ def cpu_baseline(ang_np, x_np, y_np):
     g = O.Geom2D(N, N, D, PIX, PIX, DET / SDD, SID, SDD)
     mode = "ref" if O.ref_available() else "f32"
     xs, ys = x_np[:2], y_np[:2]
-    O.op2d("arc", "proj", xs[:1], ang_np, g, mode)  # warm-up
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        O.op2d("arc", "proj", xs, ang_np, g, mode)
-        O.op2d("arc", "proj_t", ys, ang_np, g, mode)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    hw = os.cpu_count() or 1
-    threads = min(math.ceil(V / 128), hw) if mode == "ref" else 1
+    t = _cpu_time(lambda: (O.op2d("arc", "proj", xs, ang_np, g, mode),
+                           O.op2d("arc", "proj_t", ys, ang_np, g, mode)))
+    threads, hw = _ref_threads(V, mode)
     return {
         "value": 2 * V * D / t, "unit": "detector samples/s", "cores": threads,
+        "hardware_concurrency": hw,
         "kind": "reference" if mode == "ref" else "port",
         "sample": (f"2 slices of the 512^2 fan-arc workload (720 views x 736 bins), "
                    f"Proj + ProjTrans, median of 3 runs ({t:.2f} s each); deeprecon's CPU "
                    f"thread policy min(ceil(V/128), cores) = {threads} threads on a "
                    f"{hw}-CPU host"),
     }
+
+
+def parallel_c0(torch, dev, check):
+    """The reference's own CPU case: parallel beam, 256x256 image of 10 seeded
+    ellipses (1.0 mm pixels), 180 views over [0, pi), 367 bins of 1.0 mm,
+    batch 1; Proj + ProjTrans by the reference CPU code (its thread policy:
+    ceil(180/128) = 2 threads) and by one plain GPU call each."""
+    from oracle import oracle as O
+    from pridediff_amd import load_radontorch
+    from pridediff_amd.phantom import ellipses2d
+
+    rt = load_radontorch()
+    n, d, v = 256, 367, 180
+    g = O.Geom2D(n, n, d, 1.0, 1.0, 1.0, 1.0, 1.0)
+    ang_np = (np.arange(v, dtype=np.float32) * np.float32(np.pi / v)).astype(np.float32)
+    x_np = ellipses2d(n, 1, count=10, seed=0)
+    opt = torch.tensor(g.options(), dtype=torch.float32)
+    ang = torch.from_numpy(ang_np).to(dev)
+    x = torch.from_numpy(x_np)[:, None].to(dev)
+
+    def gpu():
+        return rt.ProjTransParaDis2d(rt.ProjParaDis2d(x, opt, ang), opt, ang)
+    t_gpu = timed(torch, gpu, 50, 5)
+    mode = "ref" if O.ref_available() else "f32"
+    sino_np = O.op2d("para", "proj", x_np, ang_np, g, mode)
+    t_cpu = _cpu_time(lambda: (O.op2d("para", "proj", x_np, ang_np, g, mode),
+                               O.op2d("para", "proj_t", sino_np, ang_np, g, mode)), runs=5)
+    threads, hw = _ref_threads(v, mode)
+    out = {"workload": "parallel 256x256, 180 views over [0, pi), 367 bins, batch 1, Proj + ProjTrans",
+           "gpu_ms": t_gpu * 1e3, "gpu_samples_per_s": v * d / t_gpu,
+           "cpu_ms": t_cpu * 1e3, "cpu_samples_per_s": v * d / t_cpu,
+           "cpu_kind": "reference" if mode == "ref" else "port", "cpu_threads": threads,
+           "hardware_concurrency": hw}
+    if check:
+        with torch.no_grad():
+            fp = rt.ProjParaDis2d(x, opt, ang)[:, 0].cpu().numpy()
+            y = torch.from_numpy(sino_np.astype(np.float32))[:, None].to(dev)
+            pt = rt.ProjTransParaDis2d(y, opt, ang)[:, 0].cpu().numpy()
+        out["accuracy"] = _accuracy_vs(O, "para", "proj", x_np, fp, ang_np, g)
+        out["accuracy"].update(_accuracy_vs(O, "para", "proj_t", sino_np, pt, ang_np, g))
+    return out
+
+
+def _accuracy_vs(O, geom, op, inp, got, ang_np, g):
+    """Distance of a GPU result from the FP64 oracle and from the reference's
+    own CPU output, with the reference's own distance from FP64 (its floor)."""
+    f64 = O.op2d(geom, op, inp, ang_np, g, "f64")
+    out = {f"{op}_rel_l2_vs_fp64": O.rel_l2(got, f64), f"{op}_max_abs_vs_fp64": O.max_abs(got, f64)}
+    if O.ref_available():
+        ref = O.op2d(geom, op, inp, ang_np, g, "ref")
+        out[f"{op}_rel_l2_vs_reference"] = O.rel_l2(got, ref)
+        out[f"{op}_max_abs_vs_reference"] = O.max_abs(got, ref)
+        out[f"{op}_reference_floor_rel_l2_vs_fp64"] = O.rel_l2(ref, f64)
+    return out
+
+
+def c2_cpu_sample(shapes_sel):
+    """Reference CPU code on a few of the uneven instances: Proj + ProjTrans
+    per instance with its thread policy; returns per-instance samples/s."""
+    from oracle import oracle as O
+    from pridediff_amd.phantom import ellipses2d
+
+    mode = "ref" if O.ref_available() else "f32"
+    rows = []
+    for i, (h, w, v, d) in shapes_sel:
+        g = O.Geom2D(w, h, d, PIX, PIX, DET / SDD, SID, SDD)
+        a = (np.arange(v, dtype=np.float32) * np.float32(2 * np.pi / v)).astype(np.float32)
+        x = ellipses2d(w, 1, seed=3000 + i, scale=0.04)
+        y = O.op2d("arc", "proj", x, a, g, mode)
+        t = _cpu_time(lambda: (O.op2d("arc", "proj", x, a, g, mode),
+                               O.op2d("arc", "proj_t", y, a, g, mode)))
+        threads, hw = _ref_threads(v, mode)
+        rows.append({"instance": i, "image": [h, w], "views": v, "bins": d, "seconds": t,
+                     "samples_per_s": v * d / t, "threads": threads})
+    tot_s = sum(r["seconds"] for r in rows)
+    tot_samples = sum(r["views"] * r["bins"] for r in rows)
+    return {"value": tot_samples / tot_s, "unit": "detector samples/s",
+            "cores": max(r["threads"] for r in rows),
+            "kind": "reference" if mode == "ref" else "port",
+            "sample": (f"{len(rows)} of the 64 instances (lightest, middle, heaviest by V*N^2), "
+                       "Proj + ProjTrans each, median of 3; deeprecon's thread policy per "
+                       "instance"),
+            "instances": rows, "hardware_concurrency": os.cpu_count() or 1}
 
 
 def main():
@@ -401,7 +594,8 @@ def main():
 
     if args.workload == "uneven":
         dev = torch.device("cuda:0")
-        c2 = uneven_c2(torch, dev, args.steps, args.warmup, not args.no_accuracy, not args.no_tune)
+        c2 = uneven_c2(torch, dev, args.steps, args.warmup, not args.no_accuracy, not args.no_tune,
+                       cpu=not args.no_cpu)
         samples = c2.pop("detector_samples")
         print(json.dumps({
             "metric": "FP+BP detector samples/s & %HBM roofline, 64 uneven fan-arc instances",
@@ -412,10 +606,8 @@ def main():
             "data": "synthetic (seeded ellipse phantoms)",
             "config": {"workload": "64 fan-arc instances, N in {256,384,512}, V in [96,720], "
                                    "D = ceil(1.44 N), one launch per op"},
-            "roofline": {"bound": "hbm", "achieved": c2["algorithmic_GB"] / c2["batched_ms"] * 1e3,
-                         "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": c2["roofline_frac"],
-                         "traffic": None, "kernel": "step"},
-            "uneven_c2": c2, "cpu_baseline": None}), flush=True)
+            "roofline": c2.pop("roofline"),
+            "uneven_c2": c2, "cpu_baseline": c2.pop("cpu_sample", None)}), flush=True)
         return
     if args.workload == "fidelity":
         print(json.dumps(bench_fidelity(torch, args.steps, args.warmup, not args.no_accuracy)),
@@ -423,7 +615,7 @@ def main():
         return
     if args.workload != "fanarc":
         print(json.dumps(bench3d(torch, args.workload, args.steps, args.warmup,
-                                 not args.no_accuracy)), flush=True)
+                                 not args.no_accuracy, dc_loop=not args.no_dc_loop)), flush=True)
         return
     from pridediff_amd import load_radontorch
     from pridediff_amd.autotune import tune_uniform
@@ -474,16 +666,12 @@ def main():
            rt.ADJ_TILE8: "k_pixels_tile"}.get(BP.variant, "k_pixels")
     kernels = {
         "k_bins_split + k_bins_reduce (A, image->sino)":
-            {"ms": fp_ms, "GB/s": b_alg / (fp_ms * 1e-3) / 1e9},
-        f"{adj} (A^T, sino->image)": {"ms": bp_ms, "GB/s": b_alg / (bp_ms * 1e-3) / 1e9},
+            {"ms": fp_ms, "effective_GBps": b_alg / (fp_ms * 1e-3) / 1e9},
+        f"{adj} (A^T, sino->image)": {"ms": bp_ms, "effective_GBps": b_alg / (bp_ms * 1e-3) / 1e9},
     }
     dom_ms = max(fp_ms, bp_ms)
-    dom_name = "k_bins_split" if fp_ms >= bp_ms else adj
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "fanarc_c1_hbm_traffic.json")
-    if os.path.exists(pmc):
-        with open(pmc) as f:
-            traffic = json.load(f).get(dom_name)
+    dom_name = (f"k_bins_split<{BP.tile},0>" if fp_ms >= bp_ms else f"{adj}<{BP.tile},0>")
+    roofline = roofline_c1(dom_name, dom_ms, b_alg, t_batched)
 
     if args.copy_probe:
         src = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -502,14 +690,28 @@ def main():
         g = O.Geom2D(N, N, D, PIX, PIX, DET / SDD, SID, SDD)
         with torch.no_grad():
             fp = BP.projection(x0)[:1, 0].cpu().numpy()
-        want = O.op2d("arc", "proj", x0[:1, 0].cpu().numpy(), ang_np, g, "f64")
-        acc["fp_rel_l2_vs_fp64"] = O.rel_l2(fp, want)
-        acc["fp_max_abs_vs_fp64"] = O.max_abs(fp, want)
+            yt = torch.rand(1, 1, V, D, device=dev, generator=torch.Generator(device=dev).manual_seed(3))
+            yt = yt.expand(B, 1, V, D).contiguous()
+            pt = BP.projection_t(yt)[:1, 0].cpu().numpy()
+        acc.update(_accuracy_vs(O, "arc", "proj", x0[:1, 0].cpu().numpy(), fp, ang_np, g))
+        acc.update(_accuracy_vs(O, "arc", "proj_t", yt[:1, 0].cpu().numpy(), pt, ang_np, g))
+        acc["note"] = ("one slice of the batch-16 planned path; the reference's own FP32 CPU "
+                       "code is *_reference_floor from FP64 (sequential FP32 pixel-edge "
+                       "accumulation, FanArcDisCPU2d.cpp:131)")
 
     c2 = None if args.no_c2 else uneven_c2(torch, dev, max(3, args.steps // 2), 2,
-                                           not args.no_accuracy, not args.no_tune)
+                                           not args.no_accuracy, not args.no_tune,
+                                           cpu=not args.no_cpu)
     if c2 is not None:
         c2.pop("detector_samples")
+    legs = {}
+    if not args.no_3d:
+        for kind in ("cone_flat", "helical"):
+            r = bench3d(torch, kind, 3, 1, not args.no_accuracy,
+                        dc_loop=(kind == "helical" and not args.no_dc_loop))
+            legs[kind] = {k: r[k] for k in ("config", "ms_per_step", "value", "roofline", "kernels",
+                                            "accuracy", "dc_loop") if k in r}
+    c0 = None if args.no_c0 else parallel_c0(torch, dev, not args.no_accuracy)
     cpu = None if args.no_cpu else cpu_baseline(ang_np, x_np, y_np)
     samples = B * V * D
     out = {
@@ -531,14 +733,13 @@ def main():
         "plain": {"ms_per_step": t_plain * 1e3, "value": samples / t_plain,
                   "batched_speedup_per_slice": t_plain / t_batched},
         "autotune": tune,
-        "roofline": {"bound": "hbm", "achieved": b_alg / (dom_ms * 1e-3) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": b_alg / (dom_ms * 1e-3) / HBM_PEAK, "traffic": traffic,
-                     "kernel": dom_name,
-                     "step_frac": 2 * b_alg / t_batched / HBM_PEAK},
+        "roofline": roofline,
         "kernels": kernels,
-        "uneven_c2": c2,
         "accuracy": acc,
+        "uneven_c2": c2,
+        "cone_flat_c3": legs.get("cone_flat"),
+        "helical_c4": legs.get("helical"),
+        "parallel_c0": c0,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

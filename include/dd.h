@@ -4,10 +4,8 @@
  * Plain pointers and sizes only: device pointers for data, host pointers for
  * geometry structs, a hipStream_t passed as void*.  Every entry point is
  * stream-ordered and returns a dd_status; none synchronises the host and none
- * allocates memory the caller can see.  Scratch space is either passed in
- * (dd*_workspace_size + the `ws` argument) or, for the reference-named entry
- * points in deeprecon_hip.h, taken from a per-device pool that the library
- * grows outside stream capture.
+ * allocates memory the caller can see.  Scratch space is passed in
+ * (dd*_workspace_size + the `ws` argument).
  *
  * Reference interface replaced: the raw-pointer C level of deeprecon,
  *   deeprecon/include/dis2d/{FanArc,FanFlat,Para}DisCUDA2d.h and
@@ -133,6 +131,9 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
 #define DD_ADJ_TILE16 1
 #define DD_ADJ_TILE8 2
 #define DD_ADJ_QUAD 3
+/* Uneven batches only: the 4 x 4 pixel blocks, each line's bins read as one
+ * window of loads instead of one bin at a time. */
+#define DD_ADJ_BLOCK_WINDOW 4
 
 /* `tile` = instances sharing one geometry evaluation (1, 2, 4, 8 or 16;
  * 0 = automatic); it must be the same in both calls.  `variant` selects the
@@ -143,6 +144,46 @@ dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
                            const void* geo, const float* input, float* output,
                            int batch, int views, int tile, int variant,
                            void* ws, size_t ws_bytes, void* stream);
+
+/* Output epilogue of a planned op, for fused update steps such as the
+ * data-consistency step x - w * FBP(A x - y) (PrideDiff's fidelity module,
+ * model/pridediff.py:56-68; no counterpart at the reference's C level):
+ *   output = alpha * s * op(input) + beta * c,
+ * s = *scale (a device scalar, e.g. the learned step weight w) or 1 when
+ * scale is NULL, c a device array shaped like the output (NULL: no term;
+ * it must not alias the output).  Both are read on the device when the op
+ * runs, so a captured graph follows later changes of *scale and of c. */
+typedef struct {
+  const float* c;
+  const float* scale;
+  float alpha;
+  float beta;
+} dd_epilogue;
+
+/* Workgroup shape of the planned gathers, the (views x detector bins x image
+ * lines) tile a workgroup covers; 0 in a field keeps the built-in choice.
+ *   fwd_waves : image -> sinogram: 63-bin wavefronts per workgroup (1, 2, 4);
+ *   fwd_views : image -> sinogram: views per workgroup, one after the other
+ *               (1 - 8); the lines are one of the 8 per-XCD line chunks;
+ *   pix_width : per-pixel sinogram -> image variant: pixels per workgroup
+ *               row (16, 32 or 64; 256 threads, so 16, 8 or 4 rows).
+ * The shape never changes a result bit; it is what the autotuner sweeps. */
+typedef struct {
+  int fwd_waves;
+  int fwd_views;
+  int pix_width;
+} dd_shape;
+
+/* dd2d_run_planned with a workgroup shape (NULL: built-in) and an epilogue
+ * applied where the op writes its output (in the forward's chunk
+ * reduction, or in the backprojection gather), so the residual A x - y and
+ * the update x - w BP(r) cost no extra pass (NULL: none).  With both NULL it
+ * is dd2d_run_planned. */
+dd_status dd2d_run_planned_ep(int scan, int op, const dd_geom2d* g,
+                              const void* geo, const float* input, float* output,
+                              int batch, int views, int tile, int variant,
+                              const dd_shape* shape, const dd_epilogue* ep, void* ws,
+                              size_t ws_bytes, void* stream);
 
 /* Batched projection of many instances of uneven geometry.
  *

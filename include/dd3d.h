@@ -42,8 +42,9 @@ typedef struct {
 
 /* Workspace bytes for one call (0 for an invalid geometry).  Geometries
  * whose per-view detector block (rows + 2) * cols * padded batch reaches
- * 2^30 floats are rejected (0 here, DD_BAD_ARG from dd3d_run): the
- * sinogram -> volume gather addresses a view's block with 32-bit offsets. */
+ * 2^29 floats are rejected (0 here, DD_BAD_ARG from dd3d_run): the
+ * sinogram -> volume gather addresses a view's block with 32-bit offsets.
+ * The size depends on dd3d_set_compensated() at the time of the query. */
 size_t dd3d_workspace_size(int scan, int op, const dd_geom3d* g, int batch,
                            int views);
 
@@ -60,6 +61,19 @@ dd_status dd3d_run(int scan, int op, const dd_geom3d* g, const float* input,
                    float* output, const float* angles, const float* source_pos,
                    int batch, int views, void* ws, size_t ws_bytes,
                    void* stream);
+
+/* Running sums over detector rows of the sinogram -> volume ops:
+ *   1: compensated FP32 pairs (hi + lo, plus each row's own value), so every
+ *      slice's sum is rounded relative to the slice;
+ *   0: rounded to FP32 (a third of the workspace; error relative to the
+ *      whole running sum, i.e. growing with the number of detector rows);
+ *   2 (default, and any other value): compensated when the instance tile is
+ *      1 or 2 (no extra load per evaluation), rounded at tile 4 (batches of
+ *      3 or more), where the compensated gather needs a third 16-byte load.
+ * Process-wide; affects calls issued afterwards and the workspace they need.
+ * No counterpart in the reference, whose adjoint walks rows and slices with
+ * atomics (ConeFlatDisCUDA3d.cu:284-536). */
+void dd3d_set_compensated(int mode);
 
 #ifdef __cplusplus
 }

@@ -2,10 +2,11 @@
 
 Sweeps the launch parameters of a planned operator on the problem it will
 run — the instance tile (how many instances share one geometry evaluation,
-1-16) and the sinogram -> image kernel variant (per pixel, 16x16 or 8x8
-tiles with LDS edge positions, or one thread per 2x2 pixels) of a uniform
-batch, the grid size and
-kernel variant of an uneven batch —
+1-16), the sinogram -> image kernel variant (per pixel, 16x16 or 8x8 tiles
+with LDS edge positions, or one thread per 2x2 pixels) and the workgroup
+tiles of both gathers (forward: 63-bin waves x views per workgroup over one
+per-XCD line chunk; per-pixel adjoint: 16x16, 32x8 or 64x4 pixels) of a
+uniform batch; the grid size and kernel variant of an uneven batch —
 times one forward projection plus one adjoint per candidate with HIP events,
 keeps the fastest, and remembers it in a JSON cache keyed by the device and
 the geometry, so later runs skip the sweep.
@@ -15,6 +16,7 @@ Cache file: ``$PRIDEDIFF_AMD_TUNE_CACHE`` or
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import statistics
@@ -27,13 +29,29 @@ CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
 
 
-KEY_VERSION = 6  # bumped whenever the kernels or the sweep change
+SWEEP_VERSION = 7  # bumped when the set of swept parameters changes
+
+_LIB_DIGEST = None
+
+
+def _lib_digest() -> str:
+    """Digest of the built libdd.so: any rebuild of the kernels invalidates
+    every cached choice, with no version to remember to bump."""
+    global _LIB_DIGEST
+    if _LIB_DIGEST is None:
+        lib = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdd.so")
+        h = hashlib.sha1()
+        with open(lib, "rb") as f:
+            for block in iter(lambda: f.read(1 << 20), b""):
+                h.update(block)
+        _LIB_DIGEST = h.hexdigest()[:16]
+    return _LIB_DIGEST
 
 
 def _device_key() -> str:
     p = torch.cuda.get_device_properties(torch.cuda.current_device())
     arch = getattr(p, "gcnArchName", "") or p.name
-    return f"v{KEY_VERSION}|{arch}|{p.multi_processor_count}cu"
+    return f"s{SWEEP_VERSION}|lib:{_lib_digest()}|{arch}|{p.multi_processor_count}cu"
 
 
 def _load():
@@ -70,11 +88,18 @@ def time_ms(fn, reps: int = 5, warmup: int = 2) -> float:
     return statistics.median(times)
 
 
+FWD_SHAPES = ((4, 1), (2, 1), (1, 1), (4, 2), (2, 2), (1, 4))  # (waves, views) per workgroup
+PIX_WIDTHS = (16, 32, 64)                                        # per-pixel adjoint workgroups
+
+
 def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8, 16),
-                 variants=None, use_cache: bool = True,
+                 variants=None, use_cache: bool = True, shapes: bool = True,
                  report: dict | None = None) -> BatchedProjector2d:
-    """Fastest ``BatchedProjector2d`` (tile, kernel variant) for this
-    projector, angle set and batch."""
+    """Fastest ``BatchedProjector2d`` for this projector, angle set and batch:
+    first the instance tile x adjoint kernel variant, then, for the winner,
+    the workgroup tile of each gather — forward (63-bin waves x views per
+    workgroup; its lines are one of the 8 per-XCD chunks) and per-pixel
+    adjoint (pixels per workgroup row)."""
     if variants is None:
         variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD)
     opts = [round(float(v), 6) for v in projector.host_options().tolist()]
@@ -84,7 +109,8 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
         if report is not None:
             report.update(table[key])
         return BatchedProjector2d(projector, angles, batch, tile=table[key]["tile"],
-                                  variant=table[key]["variant"])
+                                  variant=table[key]["variant"],
+                                  shape=tuple(table[key].get("shape", (0, 0, 0))))
     h, w = projector.image_size[1], projector.image_size[0]
     x = torch.rand(batch, 1, h, w, device=angles.device)
     y = torch.rand(batch, 1, angles.numel(), projector.num_det, device=angles.device)
@@ -99,23 +125,45 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
                                                      bp.plan.run(rt.PROJ_T, y)))
             del bp
     best = min(results, key=results.get)
-    entry = {"tile": best[0], "variant": best[1], "ms": results[best],
-             "swept": {f"tile{k[0]}/var{k[1]}": v for k, v in results.items()}}
+    swept = {f"tile{k[0]}/var{k[1]}": v for k, v in results.items()}
+    shape, best_ms = (0, 0, 0), results[best]
+    if shapes:
+        def timed_shape(sh):
+            bp = BatchedProjector2d(projector, angles, batch, tile=best[0], variant=best[1],
+                                    shape=sh)
+            with torch.no_grad():
+                ms = time_ms(lambda: (bp.plan.run(rt.PROJ, x), bp.plan.run(rt.PROJ_T, y)))
+            del bp
+            return ms
+        fwd = {}
+        for wv in FWD_SHAPES:
+            fwd[wv] = timed_shape((wv[0], wv[1], 0))
+            swept[f"tile{best[0]}/var{best[1]}/fwd{wv[0]}x{wv[1]}"] = fwd[wv]
+        wv = min(fwd, key=fwd.get)
+        pix = {0: fwd[wv]}
+        if best[1] == rt.ADJ_PER_PIXEL:
+            for pw in PIX_WIDTHS:
+                pix[pw] = timed_shape((wv[0], wv[1], pw))
+                swept[f"tile{best[0]}/var{best[1]}/fwd{wv[0]}x{wv[1]}/pix{pw}"] = pix[pw]
+        pw = min(pix, key=pix.get)
+        shape, best_ms = (wv[0], wv[1], pw), pix[pw]
+    entry = {"tile": best[0], "variant": best[1], "shape": list(shape), "ms": best_ms,
+             "swept": swept}
     if use_cache:
         table = _load()
         table[key] = entry
         _store(table)
     if report is not None:
         report.update(entry)
-    return BatchedProjector2d(projector, angles, batch, tile=best[0], variant=best[1])
+    return BatchedProjector2d(projector, angles, batch, tile=best[0], variant=best[1], shape=shape)
 
 
-def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(2, 8, 32, 128, 512), variants=None,
+def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(0, 2, 8, 32, 128, 512), variants=None,
                 use_cache: bool = True, report: dict | None = None) -> UnevenBatch2d:
     """Sets the fastest grid size and kernel variant on ``batch``
-    (in place)."""
+    (in place).  Grid 0 is one workgroup per work item (the default)."""
     if variants is None:
-        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD)
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD, rt.ADJ_BLOCK_WINDOW)
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
     sig = sorted(batch.shapes)
     key = f"{_device_key()}|uneven|{batch.scan_type}|{sig}"

@@ -36,6 +36,10 @@ class BatchedProjector2d(torch.nn.Module):
     """All six 2D ops of ``projector`` for a fixed angle set and batch size.
 
     ``projector`` is a ``projector2d``; ``batch`` counts B*C instances.
+    ``tile``, ``variant`` and ``shape`` = (forward waves per workgroup,
+    forward views per workgroup, adjoint pixel-tile width; 0 = built-in)
+    are launch choices (``autotune.tune_uniform`` sweeps them); none changes
+    a result bit.
     """
 
     _PAIRS = {"projection": ("PROJ", "PROJ_T"), "projection_t": ("PROJ_T", "PROJ"),
@@ -43,15 +47,17 @@ class BatchedProjector2d(torch.nn.Module):
               "backprojection_w": ("BPROJ_W", "BPROJ_TW")}
 
     def __init__(self, projector, angles: torch.Tensor, batch: int, tile: int = 0,
-                 variant: int = -1):
+                 variant: int = -1, shape=(0, 0, 0)):
         super().__init__()
         self.scan_type = projector.scan_type
+        fwd_waves, fwd_views, pix_width = shape
         self.plan = rt.Plan2d(rt.SCAN[projector.scan_type], projector.host_options(),
                               angles.contiguous().to(torch.float32), int(batch), int(tile),
-                              int(variant))
+                              int(variant), int(fwd_waves), int(fwd_views), int(pix_width))
         self.batch = int(batch)
         self.tile = self.plan.tile
         self.variant = self.plan.variant
+        self.shape = tuple(self.plan.shape)
 
     def _op(self, name, x):
         a, b = self._PAIRS[name]

@@ -67,6 +67,35 @@ constexpr int pix_tile_w() {
   return BT >= 8 ? 64 : 16;
 }
 
+// Workgroup shape of the planned gathers (dd_shape, validated): forward
+// waves per workgroup and views per workgroup; per-pixel adjoint workgroup
+// width (0 = pix_tile_w).  None of them changes a result bit.
+struct Shape {
+  int fwd_waves = kThreads / 64;
+  int fwd_views = 1;
+  int pix_width = 0;
+};
+
+bool make_shape(const dd_shape* in, Shape* out) {
+  Shape s;
+  if (in) {
+    if (in->fwd_waves) {
+      if (in->fwd_waves != 1 && in->fwd_waves != 2 && in->fwd_waves != 4) return false;
+      s.fwd_waves = in->fwd_waves;
+    }
+    if (in->fwd_views) {
+      if (in->fwd_views < 1 || in->fwd_views > 8) return false;
+      s.fwd_views = in->fwd_views;
+    }
+    if (in->pix_width) {
+      if (in->pix_width != 16 && in->pix_width != 32 && in->pix_width != 64) return false;
+      s.pix_width = in->pix_width;
+    }
+  }
+  *out = s;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // one-call kernels: thin grids over the shared bodies
 // ---------------------------------------------------------------------------
@@ -101,25 +130,15 @@ __global__ __launch_bounds__(kThreads) void k_interleave_sino(
 // Writes unweighted partial sums part[chunk][b][v][k] (walk order);
 // k_bins_reduce adds them in chunk order.
 
+// One view of k_bins_split: bin k's unweighted sum over the chunk's lines.
+// Every lane stays to the end (the edge exchange needs the whole wave);
+// lanes past the last bin work on clamped edges and write nothing.
 template <int BT, int WM>
-__global__ __launch_bounds__(kThreads) void k_bins_split(
-    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+__device__ __forceinline__ void bins_split_view(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const float* __restrict__ xrow, const float* __restrict__ xcol,
-    float* __restrict__ part) {
-  constexpr int kBinsPerBlock = (kThreads / 64) * kBinsPerWave;
-  const int kb_n = (g.D + kBinsPerBlock - 1) / kBinsPerBlock;
-  const int chunk = blockIdx.x % kSplit;
-  int rest = blockIdx.x / kSplit;
-  const int kb = rest % kb_n;
-  rest /= kb_n;
-  const int v = rest % g.V;
-  const int b0 = (rest / g.V) * BT;
-  const int lane = static_cast<int>(threadIdx.x % 64);
-  const int k = kb * kBinsPerBlock + static_cast<int>(threadIdx.x / 64) * kBinsPerWave + lane;
-  const bool edge_lane = lane == 63;
-  // every lane stays to the end (the edge exchange needs the whole wave);
-  // lanes past the last bin work on clamped edges and write nothing
+    const float* __restrict__ xrow, const float* __restrict__ xcol, float* __restrict__ part,
+    int v, int chunk, int k, int b0, bool edge_lane) {
   const ViewGeo vg = views[v];
   const int L = vg.xaxis ? g.H : g.W;
   const int N = vg.xaxis ? g.W : g.H;
@@ -142,12 +161,38 @@ __global__ __launch_bounds__(kThreads) void k_bins_split(
       part[((static_cast<size_t>(chunk) * g.Bp + b0 + b) * g.V + v) * g.D + k] = acc[b];
 }
 
+// The workgroup is a (views x bins x lines) tile: blockDim.x / 64 waves of 63
+// bins, `vpb` consecutive views one after the other, the lines of one chunk.
+template <int BT, int WM>
+__global__ __launch_bounds__(kThreads) void k_bins_split(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float* __restrict__ xrow, const float* __restrict__ xcol,
+    float* __restrict__ part, int vpb) {
+  const int bins_per_block = static_cast<int>(blockDim.x / 64) * kBinsPerWave;
+  const int kb_n = (g.D + bins_per_block - 1) / bins_per_block;
+  const int vb_n = (g.V + vpb - 1) / vpb;
+  const int chunk = blockIdx.x % kSplit;
+  int rest = blockIdx.x / kSplit;
+  const int kb = rest % kb_n;
+  rest /= kb_n;
+  const int vb = rest % vb_n;
+  const int b0 = (rest / vb_n) * BT;
+  const int lane = static_cast<int>(threadIdx.x % 64);
+  const int k = kb * bins_per_block + static_cast<int>(threadIdx.x / 64) * kBinsPerWave + lane;
+  const bool edge_lane = lane == 63;
+  for (int v = vb * vpb; v < min((vb + 1) * vpb, g.V); ++v)
+    bins_split_view<BT, WM>(g, views, edges, lines, pint, xrow, xcol, part, v, chunk, k, b0,
+                            edge_lane);
+}
+
+
 // part [kSplit][Bp][V][D] -> out [B][V][D]: chunk sums in order, times the
 // bin's FP weight, detector order.
 template <int WM>
 __global__ __launch_bounds__(kThreads) void k_bins_reduce(
     Geo2 g, const ViewGeo* __restrict__ views, const float* __restrict__ wfp,
-    const float* __restrict__ part, float* __restrict__ out) {
+    const float* __restrict__ part, float* __restrict__ out, Epi ep) {
   const int kb_n = (g.D + kThreads - 1) / kThreads;
   const int k = (blockIdx.x % kb_n) * kThreads + threadIdx.x;
   const int v = (blockIdx.x / kb_n) % g.V, b = blockIdx.x / (kb_n * g.V);
@@ -160,24 +205,25 @@ __global__ __launch_bounds__(kThreads) void k_bins_reduce(
   const ViewGeo vg = views[v];
   const int kk = vg.flip ? g.D - 1 - k : k;
   const float wk = (WM == W_FP) ? wfp[static_cast<size_t>(v) * g.D + k] : 1.0f;
-  out[(static_cast<size_t>(b) * g.V + v) * g.D + kk] = sum * wk;
+  const size_t o = (static_cast<size_t>(b) * g.V + v) * g.D + kk;
+  out[o] = epi_value(ep, sum * wk, o);
 }
 
 // sinogram -> image: one thread per pixel, BT instances, all views.
 // Variant DD_ADJ_PER_PIXEL: every pixel finds its bins on its own.
-template <int BT, int WM>
+template <int BT, int WM, int TW>
 __global__ __launch_bounds__(kThreads) void k_pixels(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const float* __restrict__ sino, float* __restrict__ out) {
-  constexpr int TW = pix_tile_w<BT>(), TH = kThreads / TW;
+    const float* __restrict__ sino, float* __restrict__ out, Epi ep) {
+  constexpr int TH = kThreads / TW;
   const int c = blockIdx.x * TW + (threadIdx.x % TW);
   // with 64-wide rows a wave is one image row: say so, for scalar row loads
   const int ty = TW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x / TW)
                           : static_cast<int>(threadIdx.x / TW);
   const int r = blockIdx.y * TH + ty;
   if (c >= g.W || r >= g.H) return;
-  pixels_body<BT, WM>(g, views, edges, lines, pint, sino, out, r, c, blockIdx.z * BT);
+  pixels_body<BT, WM>(g, views, edges, lines, pint, sino, out, r, c, blockIdx.z * BT, ep);
 }
 
 // View-resolved sinogram -> image: out[b][v] is view v's contribution
@@ -213,11 +259,11 @@ template <int BT, int WM>
 __global__ __launch_bounds__(kThreads) void k_pixels_quad(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const float* __restrict__ sino, float* __restrict__ out) {
+    const float* __restrict__ sino, float* __restrict__ out, Epi ep) {
   const int c0 = blockIdx.x * (2 * kTile) + 2 * (threadIdx.x % kTile);
   const int r0 = blockIdx.y * (2 * kTile) + 2 * (threadIdx.x / kTile);
   if (c0 >= g.W || r0 >= g.H) return;
-  quads_body<BT, WM>(g, views, edges, lines, pint, sino, out, r0, c0, blockIdx.z * BT);
+  quads_body<BT, WM>(g, views, edges, lines, pint, sino, out, r0, c0, blockIdx.z * BT, ep);
 }
 
 // Variants DD_ADJ_TILE16 / DD_ADJ_TILE8: a T x T tile stages its lines' edge
@@ -226,10 +272,10 @@ template <int BT, int WM, int T>
 __global__ __launch_bounds__(T * T) void k_pixels_tile(
     Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const float* __restrict__ sino, float* __restrict__ out) {
+    const float* __restrict__ sino, float* __restrict__ out, Epi ep) {
   __shared__ PixelTileLds<T> lds;
   pixels_tile_body<BT, WM, T>(g, views, edges, lines, pint, sino, out, blockIdx.x,
-                              blockIdx.y, blockIdx.z * BT, lds);
+                              blockIdx.y, blockIdx.z * BT, lds, ep);
 }
 
 // ---------------------------------------------------------------------------
@@ -385,10 +431,27 @@ void launch_setup(const Geo2& g, const float* angles, void* geo,
                      reinterpret_cast<float*>(b + L.pint));
 }
 
+// Per-pixel adjoint of one workgroup width (256 threads: TW x 256 / TW pixels).
+template <int BT, int TW>
+void launch_pixels(const Geo2& g, int op, const GeoPtrs& G, const float* buf0, float* output,
+                   hipStream_t st, const Epi& ep) {
+  constexpr int TH = kThreads / TW;
+  dim3 grid((g.W + TW - 1) / TW, (g.H + TH - 1) / TH, g.Bp / BT);
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_pixels<BT, W_FP, TW>), grid, dim3(kThreads), 0, st, g, G.views,
+                       G.edges, G.lines, G.pint, buf0, output, ep);
+  else if (op == DD_BPROJ)
+    hipLaunchKernelGGL((k_pixels<BT, W_BP, TW>), grid, dim3(kThreads), 0, st, g, G.views,
+                       G.edges, G.lines, G.pint, buf0, output, ep);
+  else
+    hipLaunchKernelGGL((k_pixels<BT, W_BPW, TW>), grid, dim3(kThreads), 0, st, g, G.views,
+                       G.edges, G.lines, G.pint, buf0, output, ep);
+}
+
 template <int BT>
 dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
                       float* output, const GeoPtrs& G, char* ws, const Layout& L,
-                      hipStream_t st) {
+                      hipStream_t st, const Epi& ep, const Shape& sh) {
   auto* buf0 = reinterpret_cast<float*>(ws + L.buf0);
   auto* buf1 = reinterpret_cast<float*>(ws + L.buf1);
   const int ntile = g.Bp / BT;
@@ -398,25 +461,27 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
                        dim3(kThreads), 0, st, g, input, buf0, buf1);
     auto* part = reinterpret_cast<float*>(ws + L.part);
     const int kb_n = (g.D + kThreads - 1) / kThreads;
-    const int kbs_n = (g.D + (kThreads / 64) * kBinsPerWave - 1) / ((kThreads / 64) * kBinsPerWave);
-    dim3 pgrid(kSplit * kbs_n * g.V * ntile);
+    const int waves = sh.fwd_waves, vpb = sh.fwd_views;
+    const int kbs_n = (g.D + waves * kBinsPerWave - 1) / (waves * kBinsPerWave);
+    dim3 pgrid(kSplit * kbs_n * ((g.V + vpb - 1) / vpb) * ntile);
+    dim3 pblock(64 * waves);
     dim3 rgrid(kb_n * g.V * g.B);
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ) {
-      hipLaunchKernelGGL((k_bins_split<BT, W_FP>), pgrid, dim3(kThreads), 0, st, g, G.views,
-                         G.edges, G.lines, G.pint, buf0, buf1, part);
+      hipLaunchKernelGGL((k_bins_split<BT, W_FP>), pgrid, pblock, 0, st, g, G.views,
+                         G.edges, G.lines, G.pint, buf0, buf1, part, vpb);
       hipLaunchKernelGGL((k_bins_reduce<W_FP>), rgrid, dim3(kThreads), 0, st, g, G.views,
-                         G.wfp, part, output);
+                         G.wfp, part, output, ep);
     } else if (op == DD_BPROJ_T) {
-      hipLaunchKernelGGL((k_bins_split<BT, W_BP>), pgrid, dim3(kThreads), 0, st, g, G.views,
-                         G.edges, G.lines, G.pint, buf0, buf1, part);
+      hipLaunchKernelGGL((k_bins_split<BT, W_BP>), pgrid, pblock, 0, st, g, G.views,
+                         G.edges, G.lines, G.pint, buf0, buf1, part, vpb);
       hipLaunchKernelGGL((k_bins_reduce<W_BP>), rgrid, dim3(kThreads), 0, st, g, G.views,
-                         G.wfp, part, output);
+                         G.wfp, part, output, ep);
     } else {
-      hipLaunchKernelGGL((k_bins_split<BT, W_BPW>), pgrid, dim3(kThreads), 0, st, g, G.views,
-                         G.edges, G.lines, G.pint, buf0, buf1, part);
+      hipLaunchKernelGGL((k_bins_split<BT, W_BPW>), pgrid, pblock, 0, st, g, G.views,
+                         G.edges, G.lines, G.pint, buf0, buf1, part, vpb);
       hipLaunchKernelGGL((k_bins_reduce<W_BPW>), rgrid, dim3(kThreads), 0, st, g, G.views,
-                         G.wfp, part, output);
+                         G.wfp, part, output, ep);
     }
   } else {
     dim3 sgrid((g.D + kThreads - 1) / kThreads, g.V, ntile);
@@ -431,47 +496,43 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       dim3 grid((g.W + 2 * kTile - 1) / (2 * kTile), (g.H + 2 * kTile - 1) / (2 * kTile), ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_quad<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
       else if (op == DD_BPROJ)
         hipLaunchKernelGGL((k_pixels_quad<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
       else
         hipLaunchKernelGGL((k_pixels_quad<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
     } else if (variant == DD_ADJ_TILE16) {
       dim3 grid((g.W + 15) / 16, (g.H + 15) / 16, ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 16>), grid, dim3(256), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
       else if (op == DD_BPROJ)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_BP, 16>), grid, dim3(256), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
       else
         hipLaunchKernelGGL((k_pixels_tile<BT, W_BPW, 16>), grid, dim3(256), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
     } else if (variant == DD_ADJ_TILE8) {
       dim3 grid((g.W + 7) / 8, (g.H + 7) / 8, ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 8>), grid, dim3(64), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
       else if (op == DD_BPROJ)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_BP, 8>), grid, dim3(64), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
       else
         hipLaunchKernelGGL((k_pixels_tile<BT, W_BPW, 8>), grid, dim3(64), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+                           G.views, G.edges, G.lines, G.pint, buf0, output, ep);
     } else {
-      constexpr int TW = pix_tile_w<BT>(), TH = kThreads / TW;
-      dim3 grid((g.W + TW - 1) / TW, (g.H + TH - 1) / TH, ntile);
-      if (op == DD_PROJ_T)
-        hipLaunchKernelGGL((k_pixels<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
-      else if (op == DD_BPROJ)
-        hipLaunchKernelGGL((k_pixels<BT, W_BP>), grid, dim3(kThreads), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+      const int tw = sh.pix_width > 0 ? sh.pix_width : pix_tile_w<BT>();
+      if (tw == 64)
+        launch_pixels<BT, 64>(g, op, G, buf0, output, st, ep);
+      else if (tw == 32)
+        launch_pixels<BT, 32>(g, op, G, buf0, output, st, ep);
       else
-        hipLaunchKernelGGL((k_pixels<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
-                           G.views, G.edges, G.lines, G.pint, buf0, output);
+        launch_pixels<BT, 16>(g, op, G, buf0, output, st, ep);
     }
   }
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
@@ -479,13 +540,13 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
 
 dd_status run_tiles(const Geo2& q, int op, int variant, const float* input,
                     float* output, const GeoPtrs& G, char* w, const Layout& L,
-                    hipStream_t st) {
+                    hipStream_t st, const Epi& ep = Epi{}, const Shape& sh = Shape{}) {
   switch (q.BT) {
-    case 1: return launch_tile<1>(q, op, variant, input, output, G, w, L, st);
-    case 2: return launch_tile<2>(q, op, variant, input, output, G, w, L, st);
-    case 4: return launch_tile<4>(q, op, variant, input, output, G, w, L, st);
-    case 8: return launch_tile<8>(q, op, variant, input, output, G, w, L, st);
-    default: return launch_tile<16>(q, op, variant, input, output, G, w, L, st);
+    case 1: return launch_tile<1>(q, op, variant, input, output, G, w, L, st, ep, sh);
+    case 2: return launch_tile<2>(q, op, variant, input, output, G, w, L, st, ep, sh);
+    case 4: return launch_tile<4>(q, op, variant, input, output, G, w, L, st, ep, sh);
+    case 8: return launch_tile<8>(q, op, variant, input, output, G, w, L, st, ep, sh);
+    default: return launch_tile<16>(q, op, variant, input, output, G, w, L, st, ep, sh);
   }
 }
 
@@ -640,6 +701,32 @@ dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
   if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
   return run_tiles(q, op, adjoint_variant(variant, q), input, output, geo_ptrs(geo, L),
                    static_cast<char*>(ws), L, static_cast<hipStream_t>(stream));
+}
+
+dd_status dd2d_run_planned_ep(int scan, int op, const dd_geom2d* g,
+                              const void* geo, const float* input, float* output,
+                              int batch, int views, int tile, int variant,
+                              const dd_shape* shape, const dd_epilogue* ep, void* ws,
+                              size_t ws_bytes, void* stream) {
+  dd_status rc = check_op(scan, op);
+  if (rc != DD_OK) return rc;
+  if (!geo || !input || !output) return DD_BAD_ARG;
+  if (ep && ep->c == output) return DD_BAD_ARG;  // c is read after other threads write
+  Shape sh;
+  if (!make_shape(shape, &sh)) return DD_BAD_ARG;
+  Geo2 q;
+  if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return DD_BAD_ARG;
+  const Layout L = make_layout(q, op);
+  if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
+  Epi e;
+  if (ep) {
+    e.c = ep->c;
+    e.scale = ep->scale;
+    e.alpha = ep->alpha;
+    e.beta = ep->beta;
+  }
+  return run_tiles(q, op, adjoint_variant(variant, q), input, output, geo_ptrs(geo, L),
+                   static_cast<char*>(ws), L, static_cast<hipStream_t>(stream), e, sh);
 }
 
 }  // extern "C"

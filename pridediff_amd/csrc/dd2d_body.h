@@ -27,6 +27,23 @@ struct Geo2 {
   float dx, dy, det, sid, sdd, shx, shy, bin;
 };
 
+// Output epilogue of an op (fused update steps): out = alpha * s * result
+// + beta * c, s = *scale (a device scalar) or 1, c a device array shaped
+// like the output or null.  The default leaves the result untouched, bit for
+// bit.
+struct Epi {
+  const float* c = nullptr;
+  const float* scale = nullptr;
+  float alpha = 1.0f, beta = 0.0f;
+};
+
+__device__ __forceinline__ float epi_value(const Epi& e, float v, size_t at) {
+  if (e.scale) v *= e.alpha * *e.scale;
+  else if (e.alpha != 1.0f) v *= e.alpha;
+  if (e.c) v = fmaf(e.beta, e.c[at], v);
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // 1. per-view setup (FP64 boundary mapping)
 // ---------------------------------------------------------------------------
@@ -645,7 +662,8 @@ template <int BT, int WM>
 __device__ __forceinline__ void quads_body(
     const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const float* __restrict__ sino, float* __restrict__ out, int r0, int c0, int b0) {
+    const float* __restrict__ sino, float* __restrict__ out, int r0, int c0, int b0,
+    const Epi& ep = Epi{}) {
   const bool hr = r0 + 1 < g.H, hc = c0 + 1 < g.W;
   float xc[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, yc[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
   if constexpr (WM == W_BPW) {
@@ -712,8 +730,10 @@ __device__ __forceinline__ void quads_body(
       if ((dr == 0 || hr) && (dc == 0 || hc))
 #pragma unroll
         for (int b = 0; b < BT; ++b)
-          if (b0 + b < g.B)
-            out[(static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc] = acc[dr][dc][b];
+          if (b0 + b < g.B) {
+            const size_t at = (static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc;
+            out[at] = epi_value(ep, acc[dr][dc][b], at);
+          }
 }
 
 // Contributions of view v to the run of NP consecutive pixels ib .. ib+NP-1
@@ -758,11 +778,109 @@ __device__ __forceinline__ void pixel_run_view(
   }
 }
 
+// pixel_run_view with the bins read as one window: the K + 1 edges and K
+// sinogram values from one bin below the first pixel's guess are loaded at
+// once (independent loads, one round trip instead of one per bin), every
+// bin of the window is tested against every pixel of the run, and the walk
+// continues one bin at a time only if the run reaches past the window.  The
+// window is used only when nothing below it can reach the run (its first
+// edge lies before the run, or it starts at bin 0); otherwise the serial
+// search decides.  Per pixel the same bins in the same (ascending) order,
+// with the same overlaps: equal bit for bit.
+template <int BT, int WM, int NP, int K>
+__device__ __forceinline__ void pixel_run_view_win(
+    const Geo2& g, const ViewGeo& vg, int v, const double* __restrict__ edges,
+    const LineGeo& lg, float pl, const float* __restrict__ sino, int ib, int pf, int plst,
+    int N, int b0, const float (&gw)[NP], float (&acc)[NP][BT]) {
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (g.D + 1);
+  const int ifirst = ib + pf, ilast = ib + plst;
+  if (g.D < K + 1) {
+    pixel_run_view<BT, WM, NP>(g, vg, v, edges, lg, pl, sino, ib, pf, plst, N, b0, gw, acc);
+    return;
+  }
+  static_assert(K % 4 == 0, "window of whole 4-bin groups");
+  const int kg = guess_bin_fast(g, vg, lg.a0 + (ifirst + 0.5) * static_cast<double>(pl));
+  // edges kw .. kw + K + 1 are read in pairs, so the window ends one edge
+  // short of the table's last (index D)
+  const int kw = min(max(kg - 1, 0), g.D - K - 1);
+  double e[K + 2];
+#pragma unroll
+  for (int j = 0; j < K + 2; j += 2) {
+    // 16-byte loads of edge pairs (8-byte aligned)
+    double2 pr;
+    __builtin_memcpy(&pr, ev + kw + j, sizeof(pr));
+    e[j] = pr.x;
+    e[j + 1] = pr.y;
+  }
+  const float* __restrict__ sv = sino_view<BT>(g, sino, v, b0);
+  float q[K][BT];
+  if constexpr (BT == 1 && !sino_quad_major<BT>()) {
+    if (g.Bp == 1) {
+      // 16-byte loads of four bins (4-byte aligned)
+#pragma unroll
+      for (int j = 0; j < K; j += 4) {
+        float4 t;
+        __builtin_memcpy(&t, sv + kw + j, sizeof(t));
+        q[j][0] = t.x;
+        q[j + 1][0] = t.y;
+        q[j + 2][0] = t.z;
+        q[j + 3][0] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; ++j) load_sino<BT>(g, sv, kw + j, q[j]);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < K; ++j) load_sino<BT>(g, sv, kw + j, q[j]);
+  }
+  EdgePos p1 = edge_pos(e[0], lg.a0, lg.inv_p, N);
+  if (kw > 0 && p1.c >= ifirst) {
+    // a bin below the window may reach the run: the serial search decides
+    pixel_run_view<BT, WM, NP>(g, vg, v, edges, lg, pl, sino, ib, pf, plst, N, b0, gw, acc);
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const EdgePos p2 = edge_pos(e[j + 1], lg.a0, lg.inv_p, N);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int i = ib + p;
+      if (p >= pf && p <= plst && p1.c <= i && i <= p2.c) {
+        const float w = overlap_in_pixel(p1, p2, i) * gw[p];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) acc[p][b] = fmaf(w, q[j][b], acc[p][b]);
+      }
+    }
+    p1 = p2;
+  }
+  // past the window, one bin at a time (the walk's own loop)
+  int k = kw + K;
+  if (k >= g.D) return;
+  while (p1.c <= ilast) {
+    const EdgePos p2 = edge_pos(ev[k + 1], lg.a0, lg.inv_p, N);
+    float qk[BT];
+    load_sino<BT>(g, sv, k, qk);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int i = ib + p;
+      if (p >= pf && p <= plst && p1.c <= i && i <= p2.c) {
+        const float w = overlap_in_pixel(p1, p2, i) * gw[p];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) acc[p][b] = fmaf(w, qk[b], acc[p][b]);
+      }
+    }
+    if (++k >= g.D) break;
+    p1 = p2;
+  }
+}
+
 // sinogram -> image for the Q x Q pixel block at (r0, c0), all views: per
 // view the block is Q lines of Q consecutive pixels whichever the axis (rows
 // over columns c0 .. c0+Q-1, or columns read bottom-up).  Equal bit for bit
-// to pixels_body on each pixel.
-template <int BT, int WM, int Q>
+// to pixels_body on each pixel.  KW > 0: each line's bins are read as one
+// window of KW bins (pixel_run_view_win).
+template <int BT, int WM, int Q, int KW = 0>
 __device__ __forceinline__ void blocks_body(
     const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
@@ -803,8 +921,12 @@ __device__ __forceinline__ void blocks_body(
           gw[p] = (WM == W_FP) ? pl : kbp;
           if constexpr (WM == W_BPW) gw[p] *= fbp_weight(g, vg, xc[dr][p], yc[dr][p]);
         }
-        pixel_run_view<BT, WM, Q>(g, vg, v, edges, lv[l], pl, sino, c0, 0, nc - 1, g.W, b0, gw,
-                                  acc[dr]);
+        if constexpr (KW > 0)
+          pixel_run_view_win<BT, WM, Q, KW>(g, vg, v, edges, lv[l], pl, sino, c0, 0, nc - 1,
+                                            g.W, b0, gw, acc[dr]);
+        else
+          pixel_run_view<BT, WM, Q>(g, vg, v, edges, lv[l], pl, sino, c0, 0, nc - 1, g.W, b0,
+                                    gw, acc[dr]);
       }
     } else {
       // lines = image columns c0 + dc; pixel i = H - 1 - r, so run position p
@@ -824,8 +946,12 @@ __device__ __forceinline__ void blocks_body(
 #pragma unroll
           for (int b = 0; b < BT; ++b) run[p][b] = acc[Q - 1 - p][dc][b];
         }
-        pixel_run_view<BT, WM, Q>(g, vg, v, edges, lv[l], pl, sino, g.H - r0 - Q, Q - nr, Q - 1,
-                                  g.H, b0, gw, run);
+        if constexpr (KW > 0)
+          pixel_run_view_win<BT, WM, Q, KW>(g, vg, v, edges, lv[l], pl, sino, g.H - r0 - Q,
+                                            Q - nr, Q - 1, g.H, b0, gw, run);
+        else
+          pixel_run_view<BT, WM, Q>(g, vg, v, edges, lv[l], pl, sino, g.H - r0 - Q, Q - nr,
+                                    Q - 1, g.H, b0, gw, run);
 #pragma unroll
         for (int p = 0; p < Q; ++p)
 #pragma unroll
@@ -852,7 +978,8 @@ template <int BT, int WM>
 __device__ __forceinline__ void pixels_body(
     const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
-    const float* __restrict__ sino, float* __restrict__ out, int r, int c, int b0) {
+    const float* __restrict__ sino, float* __restrict__ out, int r, int c, int b0,
+    const Epi& ep = Epi{}) {
   float xcw = 0.0f, ycw = 0.0f;
   pixel_centre(g, 1, r, c, &xcw, &ycw);
   float acc[BT];
@@ -870,8 +997,10 @@ __device__ __forceinline__ void pixels_body(
   }
 #pragma unroll
   for (int b = 0; b < BT; ++b)
-    if (b0 + b < g.B)
-      out[(static_cast<size_t>(b0 + b) * g.H + r) * g.W + c] = acc[b];
+    if (b0 + b < g.B) {
+      const size_t at = (static_cast<size_t>(b0 + b) * g.H + r) * g.W + c;
+      out[at] = epi_value(ep, acc[b], at);
+    }
 }
 
 // Edge positions staged per (line of a pixel tile, view).
@@ -899,7 +1028,8 @@ __device__ __forceinline__ void pixels_tile_body(
     const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
     const LineGeo* __restrict__ lines, const float* __restrict__ pint,
     const float* __restrict__ sino, float* __restrict__ out, int tile_c, int tile_r,
-    int b0, PixelTileLds<kTile>& s) {
+    int b0, PixelTileLds<kTile>& s,
+    const Epi& ep = Epi{}) {
   constexpr int kEdgeCap = PixelTileLds<kTile>::kCap;
   const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
   const int c0 = tile_c * kTile, r0 = tile_r * kTile;
@@ -1004,8 +1134,10 @@ __device__ __forceinline__ void pixels_tile_body(
   if (inside) {
 #pragma unroll
     for (int b = 0; b < BT; ++b)
-      if (b0 + b < g.B)
-        out[(static_cast<size_t>(b0 + b) * g.H + r) * g.W + c] = acc[b];
+      if (b0 + b < g.B) {
+        const size_t at = (static_cast<size_t>(b0 + b) * g.H + r) * g.W + c;
+        out[at] = epi_value(ep, acc[b], at);
+      }
   }
 }
 

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "dd3d.h"
@@ -38,6 +39,18 @@ constexpr int kThreads3 = 256;
 constexpr int kColsPerBlock = 64;  // k_bins3d: 64 columns x 4 row chunks
 constexpr int kRowsPerBlock = 4;
 constexpr int kTile3 = 16;         // k_voxels3d: 16 x 16 pixels x one slice chunk
+
+// Running sums of the sinogram -> volume ops (dd3d_set_compensated):
+// 0 rounded to FP32, 1 compensated (hi/lo pairs), 2 automatic: compensated
+// when it adds no vector-memory instruction to the gather (instance tiles of
+// 1 and 2: its hi and (lo, row) tiles take the two loads the rounded form's
+// rows r and r + 1 take), rounded at tile 4, where it costs a third load.
+std::atomic<int> g_compensated{2};
+
+bool use_compensated(int bt) {
+  const int m = g_compensated.load();
+  return m == 1 || (m == 2 && bt <= 2);
+}
 
 enum WMode3 { W3_FP = 0, W3_BP = 1, W3_BPW = 2 };
 
@@ -287,13 +300,23 @@ __global__ __launch_bounds__(kThreads3) void k_interleave_volume(
 // p [B][V][R][C] -> Q [V][R+2][C][Bp]: per view and column, the running sum
 // over detector rows (columns in walk order, optionally times the FP weight
 // of each cell), Q[0] = 0, and Q[R+1] = Q[R] so an interpolation at the top
-// edge (row R, fraction 0) needs no special case.  Accumulated in FP64 and
-// stored rounded, so every entry carries one rounding and no drift.
-template <int BT, bool FOLD_W>
+// edge (row R, fraction 0) needs no special case.  Accumulated in FP64.
+//
+// COMP = false: Q stored rounded to FP32 (one rounding per entry, no drift).
+// COMP = true (compensated): Q kept as an unevaluated FP32 pair hi + lo
+// (hi = Q rounded, lo = Q - hi rounded, the exact two-sum split of the FP64
+// running sum), and beside lo the row's own value x_r, the term the sum
+// added at row r (zero at rows R and R + 1).  Planes: hi [V][R+2][C][Bp] in
+// `q`, and (lo, x) pairs [V][R+2][C][Bp][2] in `q2`.  The gather then forms
+// each slice's sum as differences of hi and of (lo + fraction x) that are no
+// larger than the slice's own rows, so its rounding error is relative to
+// the slice, not to the running sum (which, R rows deep, is up to R times
+// larger).
+template <int BT, bool FOLD_W, bool COMP>
 __global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const float* __restrict__ p,
-    float* __restrict__ q) {
+    float* __restrict__ q, float* __restrict__ q2) {
   const int nkc = (g.C + kThreads3 - 1) / kThreads3;
   const int k = (blockIdx.x % nkc) * kThreads3 + threadIdx.x;
   const int v = blockIdx.x / nkc;
@@ -311,8 +334,12 @@ __global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
   const size_t rstride = static_cast<size_t>(g.C) * g.Bp;
   float* __restrict__ qv = q + static_cast<size_t>(v) * (g.R + 2) * rstride +
                            static_cast<size_t>(k) * g.Bp + b0;
+  // (lo, x) pairs of this tile: the same element offset, doubled
+  float* __restrict__ q2v = COMP ? q2 + 2 * (static_cast<size_t>(v) * (g.R + 2) * rstride +
+                                             static_cast<size_t>(k) * g.Bp + b0)
+                                 : nullptr;
   double acc[BT];
-  float val[BT];
+  float val[BT], pair[2 * BT];
 #pragma unroll
   for (int b = 0; b < BT; ++b) {
     acc[b] = 0.0;
@@ -332,12 +359,29 @@ __global__ __launch_bounds__(kThreads3) void k_prefix_sino3d(
       const float x = (b0 + b < g.B)
                           ? p[((static_cast<size_t>(b0 + b) * g.V + v) * g.R + r) * g.C + kk]
                           : 0.0f;
-      acc[b] += static_cast<double>(x * w);
+      const float xw = x * w;
+      if constexpr (COMP) {
+        // row r's pair: lo of Q[r] (still in acc) and the row's term
+        pair[2 * b] = static_cast<float>(acc[b] - static_cast<double>(val[b]));
+        pair[2 * b + 1] = xw;
+      }
+      acc[b] += static_cast<double>(xw);
       val[b] = static_cast<float>(acc[b]);
     }
+    if constexpr (COMP) store_tile<2 * BT>(q2v + 2 * static_cast<size_t>(r) * rstride, pair);
     store_tile<BT>(qv + (r + 1) * rstride, val);
   }
   store_tile<BT>(qv + (g.R + 1) * rstride, val);
+  if constexpr (COMP) {
+    // rows R and R + 1: lo of Q[R], no row term
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      pair[2 * b] = static_cast<float>(acc[b] - static_cast<double>(val[b]));
+      pair[2 * b + 1] = 0.0f;
+    }
+    store_tile<2 * BT>(q2v + 2 * static_cast<size_t>(g.R) * rstride, pair);
+    store_tile<2 * BT>(q2v + 2 * static_cast<size_t>(g.R + 1) * rstride, pair);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -581,12 +625,13 @@ __device__ __forceinline__ int guess_col_fast(const Geo3& g, const ViewGeo3& vg,
   return static_cast<int>(kf);
 }
 
-template <int BT, int WM>
+template <int BT, int WM, bool COMP>
 __global__ __launch_bounds__(kThreads3)
 __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
-    const float* __restrict__ sino, float* __restrict__ out) {
+    const float* __restrict__ sino, const float* __restrict__ sino2,
+    float* __restrict__ out) {
   constexpr int S = slices3<BT>();
   const int ntx = (g.W + kTile3 - 1) / kTile3;
   const int col = (blockIdx.x % ntx) * kTile3 + (threadIdx.x % kTile3);
@@ -658,33 +703,67 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
         // the view's block of Q (wave-uniform base) plus 32-bit byte
         // offsets: a block holds (R + 2) C Bp < 2^30 floats (checked on the
         // host), so base + offset maps onto scalar-base vector loads
-        const char* __restrict__ qv = reinterpret_cast<const char*>(
-            sino + static_cast<size_t>(v) * (g.R + 2) * g.C * g.Bp);
+        const size_t vblock = static_cast<size_t>(v) * (g.R + 2) * g.C * g.Bp;
+        const char* __restrict__ qv = reinterpret_cast<const char*>(sino + vblock);
         const unsigned rstride = 4u * static_cast<unsigned>(g.C * g.Bp);
         const unsigned okb =
             4u * static_cast<unsigned>(k * g.Bp + b0) + static_cast<unsigned>(rbase) * rstride;
-        auto eval = [&](int j, float (&l)[BT]) {
-          // u in [-rbase, R - rbase]: rows r, r + 1 with r <= R (Q[R+1] = Q[R])
-          const float u = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(j), invb, du), ulo, uhi);
-          const float fu = floorf(u);
-          const float fr = u - fu;
-          const unsigned off = okb + static_cast<unsigned>(static_cast<int>(fu)) * rstride;
-          float q0[BT], q1[BT];
-          load_tile<BT>(reinterpret_cast<const float*>(qv + off), q0);
-          load_tile<BT>(reinterpret_cast<const float*>(qv + (off + rstride)), q1);
+        if constexpr (COMP) {
+          // Compensated form.  At slice edge u (row r = floor u, fraction f)
+          // the running sum is hi[r] + (lo[r] + f x_r); slice j receives
+          //   (hi[r'] - hi[r]) + ((lo + f x)[r'] - (lo + f x)[r])
+          // between its edges r + f and r' + f'.  Both differences are of the
+          // size of the slice's rows, so FP32 rounds them relative to the
+          // slice rather than to the R-row running sum.
+          const char* __restrict__ q2v = reinterpret_cast<const char*>(sino2 + 2 * vblock);
+          auto eval = [&](int j, float (&h)[BT], float (&m)[BT]) {
+            const float u = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(j), invb, du), ulo, uhi);
+            const float fu = floorf(u);
+            const float fr = u - fu;
+            const unsigned off = okb + static_cast<unsigned>(static_cast<int>(fu)) * rstride;
+            float lx[2 * BT];
+            load_tile<BT>(reinterpret_cast<const float*>(qv + off), h);
+            load_tile<2 * BT>(reinterpret_cast<const float*>(q2v + 2u * off), lx);
 #pragma unroll
-          for (int b = 0; b < BT; ++b) l[b] = fmaf(fr, q1[b] - q0[b], q0[b]);
-        };
-        float lprev[BT];
-        eval(0, lprev);
+            for (int b = 0; b < BT; ++b) m[b] = fmaf(fr, lx[2 * b + 1], lx[2 * b]);
+          };
+          float hprev[BT], mprev[BT];
+          eval(0, hprev, mprev);
 #pragma unroll
-        for (int j = 0; j < S; ++j) {
-          float lnext[BT];
-          eval(j + 1, lnext);
+          for (int j = 0; j < S; ++j) {
+            float hnext[BT], mnext[BT];
+            eval(j + 1, hnext, mnext);
 #pragma unroll
-          for (int b = 0; b < BT; ++b) {
-            acc[j][b] = fmaf(wxb, lnext[b] - lprev[b], acc[j][b]);
-            lprev[b] = lnext[b];
+            for (int b = 0; b < BT; ++b) {
+              acc[j][b] = fmaf(wxb, (hnext[b] - hprev[b]) + (mnext[b] - mprev[b]), acc[j][b]);
+              hprev[b] = hnext[b];
+              mprev[b] = mnext[b];
+            }
+          }
+        } else {
+          auto eval = [&](int j, float (&l)[BT]) {
+            // u in [-rbase, R - rbase]: rows r, r + 1 with r <= R (Q[R+1] = Q[R])
+            const float u = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(j), invb, du), ulo, uhi);
+            const float fu = floorf(u);
+            const float fr = u - fu;
+            const unsigned off = okb + static_cast<unsigned>(static_cast<int>(fu)) * rstride;
+            float q0[BT], q1[BT];
+            load_tile<BT>(reinterpret_cast<const float*>(qv + off), q0);
+            load_tile<BT>(reinterpret_cast<const float*>(qv + (off + rstride)), q1);
+#pragma unroll
+            for (int b = 0; b < BT; ++b) l[b] = fmaf(fr, q1[b] - q0[b], q0[b]);
+          };
+          float lprev[BT];
+          eval(0, lprev);
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            float lnext[BT];
+            eval(j + 1, lnext);
+#pragma unroll
+            for (int b = 0; b < BT; ++b) {
+              acc[j][b] = fmaf(wxb, lnext[b] - lprev[b], acc[j][b]);
+              lprev[b] = lnext[b];
+            }
           }
         }
       }
@@ -746,8 +825,10 @@ Layout3 make_layout3(const Geo3& g, int op) {
     L.buf0 = take(sizeof(float) * (vox + g.BT));
     L.buf1 = take(sizeof(float) * (vox + g.BT));
   } else {
-    L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * (g.R + 2) * g.C * g.Bp);
-    L.buf1 = L.buf0;
+    const size_t q = sizeof(float) * static_cast<size_t>(g.V) * (g.R + 2) * g.C * g.Bp;
+    L.buf0 = take(q);
+    // compensated running sums: a second plane of (lo, row value) pairs
+    L.buf1 = use_compensated(g.BT) ? take(2 * q) : L.buf0;
   }
   L.data_total = off;
   return L;
@@ -782,10 +863,38 @@ bool fill_geo3(Geo3* out, int scan, const dd_geom3d* g, int batch, int views) {
   q.shz = g->shift_z;
   q.csh = g->col_shift;
   q.rsh = g->row_shift;
-  // the adjoint addresses one view's row running sums with 32-bit byte offsets
-  if (static_cast<long long>(q.R + 2) * q.C * q.Bp >= (1LL << 30)) return false;
+  // the adjoint addresses one view's row running sums with 32-bit byte
+  // offsets (twice the offset for the (lo, row value) plane)
+  if (static_cast<long long>(q.R + 2) * q.C * q.Bp >= (1LL << 29)) return false;
   *out = q;
   return true;
+}
+
+// sinogram -> volume: running sums over rows, then the voxel gather
+template <int BT, bool COMP>
+void launch_adjoint3(const Geo3& g, int op, const float* input, float* output,
+                     const ViewGeo3* views, const double* cedge, const double* mag,
+                     const LineGeo3* lines, float* q, float* q2, hipStream_t st) {
+  dim3 sgrid((g.C + kThreads3 - 1) / kThreads3 * g.V, 1, g.Bp / BT);
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_prefix_sino3d<BT, true, COMP>), sgrid, dim3(kThreads3), 0, st, g,
+                       views, cedge, mag, input, q, q2);
+  else
+    hipLaunchKernelGGL((k_prefix_sino3d<BT, false, COMP>), sgrid, dim3(kThreads3), 0, st, g,
+                       views, cedge, mag, input, q, q2);
+  const int ntx = (g.W + kTile3 - 1) / kTile3, nty = (g.H + kTile3 - 1) / kTile3;
+  constexpr int S = slices3<BT>();
+  dim3 grid(ntx * nty, (g.Dz + S - 1) / S, g.Bp / BT);
+  ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_voxels3d<BT, W3_FP, COMP>), grid, dim3(kThreads3), 0, st, g, views,
+                       cedge, mag, lines, q, q2, output);
+  else if (op == DD_BPROJ)
+    hipLaunchKernelGGL((k_voxels3d<BT, W3_BP, COMP>), grid, dim3(kThreads3), 0, st, g, views,
+                       cedge, mag, lines, q, q2, output);
+  else
+    hipLaunchKernelGGL((k_voxels3d<BT, W3_BPW, COMP>), grid, dim3(kThreads3), 0, st, g, views,
+                       cedge, mag, lines, q, q2, output);
 }
 
 template <int BT>
@@ -816,27 +925,10 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
     else
       hipLaunchKernelGGL((k_bins3d<BT, W3_BPW>), grid, dim3(kThreads3), 0, st, g,
                          views, cedge, mag, lines, buf0, buf1, output);
+  } else if (buf1 != buf0) {
+    launch_adjoint3<BT, true>(g, op, input, output, views, cedge, mag, lines, buf0, buf1, st);
   } else {
-    dim3 sgrid((g.C + kThreads3 - 1) / kThreads3 * g.V, 1, ntile);
-    if (op == DD_PROJ_T)
-      hipLaunchKernelGGL((k_prefix_sino3d<BT, true>), sgrid, dim3(kThreads3), 0,
-                         st, g, views, cedge, mag, input, buf0);
-    else
-      hipLaunchKernelGGL((k_prefix_sino3d<BT, false>), sgrid, dim3(kThreads3), 0,
-                         st, g, views, cedge, mag, input, buf0);
-    const int ntx = (g.W + kTile3 - 1) / kTile3, nty = (g.H + kTile3 - 1) / kTile3;
-    constexpr int S = slices3<BT>();
-    dim3 grid(ntx * nty, (g.Dz + S - 1) / S, ntile);
-    ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (op == DD_PROJ_T)
-      hipLaunchKernelGGL((k_voxels3d<BT, W3_FP>), grid, dim3(kThreads3), 0, st, g,
-                         views, cedge, mag, lines, buf0, output);
-    else if (op == DD_BPROJ)
-      hipLaunchKernelGGL((k_voxels3d<BT, W3_BP>), grid, dim3(kThreads3), 0, st, g,
-                         views, cedge, mag, lines, buf0, output);
-    else
-      hipLaunchKernelGGL((k_voxels3d<BT, W3_BPW>), grid, dim3(kThreads3), 0, st, g,
-                         views, cedge, mag, lines, buf0, output);
+    launch_adjoint3<BT, false>(g, op, input, output, views, cedge, mag, lines, buf0, buf1, st);
   }
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
 }
@@ -847,6 +939,10 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
 using namespace dd;
 
 extern "C" {
+
+void dd3d_set_compensated(int mode) {
+  g_compensated = (mode == 0 || mode == 1) ? mode : 2;
+}
 
 size_t dd3d_workspace_size(int scan, int op, const dd_geom3d* g, int batch,
                            int views) {

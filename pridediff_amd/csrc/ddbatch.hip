@@ -260,9 +260,11 @@ __global__ __launch_bounds__(kThreads) void k_batch_pixels(
   }
 }
 
-// Block items: (instance, 64 x 64 pixel tile), one thread per 4 x 4 pixels.
-template <int WM>
-__global__ __launch_bounds__(kQuadThreads) void k_batch_quads(
+// Block items: (instance, 64 x 64 pixel tile), one thread per 4 x 4 pixels;
+// KW > 0 reads each line's bins as one window of KW bins.
+template <int WM, int KW>
+__global__ __launch_bounds__(kQuadThreads)
+__attribute__((amdgpu_waves_per_eu(KW > 0 ? 4 : 1))) void k_batch_quads(
     char* __restrict__ mem, int n_inst, int total, float* __restrict__ out) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int i = find_inst(mem, n_inst, kGatherQuads, item);
@@ -275,8 +277,8 @@ __global__ __launch_bounds__(kQuadThreads) void k_batch_quads(
     const int c0 = (t % tx) * kQuadTile + kBlockQ * (threadIdx.x % kQuadSide);
     const int r0 = (t / tx) * kQuadTile + kBlockQ * (threadIdx.x / kQuadSide);
     if (c0 < d.g.W && r0 < d.g.H)
-      blocks_body<kBT, WM, kBlockQ>(d.g, d.views, d.edges, d.lines, d.pint, d.sino,
-                                    out + d.img_off, r0, c0, b0);
+      blocks_body<kBT, WM, kBlockQ, KW>(d.g, d.views, d.edges, d.lines, d.pint, d.sino,
+                                        out + d.img_off, r0, c0, b0);
   }
 }
 
@@ -312,6 +314,24 @@ size_t take(size_t* off, size_t bytes) {
 int grid_for(const dd2d_batch* b, int total) {
   const int g = b->grid > 0 ? b->grid : total;
   return std::max(1, std::min(g, total));
+}
+
+// Bins per window of DD_ADJ_BLOCK_WINDOW: a 4-pixel run meets up to ~6 bins
+// of these geometries, so 8 rarely leaves a bin to the serial tail.
+constexpr int kBlockWindow = 8;
+
+template <int KW>
+void launch_quads(const dd2d_batch* b, int op, char* inst, int n, int qt, float* output,
+                  hipStream_t st) {
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_batch_quads<W_FP, KW>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0,
+                       st, inst, n, qt, output);
+  else if (op == DD_BPROJ)
+    hipLaunchKernelGGL((k_batch_quads<W_BP, KW>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0,
+                       st, inst, n, qt, output);
+  else
+    hipLaunchKernelGGL((k_batch_quads<W_BPW, KW>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0,
+                       st, inst, n, qt, output);
 }
 
 }  // namespace
@@ -409,8 +429,10 @@ void dd2d_batch_set_grid(dd2d_batch* b, int workgroups) {
 
 void dd2d_batch_set_variant(dd2d_batch* b, int variant) {
   if (b)
-    b->variant = (variant == DD_ADJ_TILE16 || variant == DD_ADJ_QUAD) ? variant
-                                                                       : DD_ADJ_PER_PIXEL;
+    b->variant = (variant == DD_ADJ_TILE16 || variant == DD_ADJ_QUAD ||
+                  variant == DD_ADJ_BLOCK_WINDOW)
+                     ? variant
+                     : DD_ADJ_PER_PIXEL;
 }
 
 dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* stream) {
@@ -477,17 +499,12 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
                          dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutSino),
                          b->total[kLayoutSino], input);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (b->variant == DD_ADJ_QUAD) {
+    if (b->variant == DD_ADJ_QUAD || b->variant == DD_ADJ_BLOCK_WINDOW) {
       const int qt = b->total[kGatherQuads];
-      if (op == DD_PROJ_T)
-        hipLaunchKernelGGL((k_batch_quads<W_FP>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0, st,
-                           inst, n, qt, output);
-      else if (op == DD_BPROJ)
-        hipLaunchKernelGGL((k_batch_quads<W_BP>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0, st,
-                           inst, n, qt, output);
+      if (b->variant == DD_ADJ_QUAD)
+        launch_quads<0>(b, op, inst, n, qt, output, st);
       else
-        hipLaunchKernelGGL((k_batch_quads<W_BPW>), dim3(grid_for(b, qt)), dim3(kQuadThreads), 0, st,
-                           inst, n, qt, output);
+        launch_quads<kBlockWindow>(b, op, inst, n, qt, output, st);
       return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
     }
     const int tot = b->total[kGatherPixels];

@@ -335,10 +335,18 @@ torch::Tensor op3d(int scan, int op, const torch::Tensor& input,
 class Plan2d {
  public:
   Plan2d(int scan, const torch::Tensor& options, const torch::Tensor& angles,
-         int64_t batch, int64_t tile, int64_t variant)
+         int64_t batch, int64_t tile, int64_t variant, int64_t fwd_waves, int64_t fwd_views,
+         int64_t pix_width)
       : scan_(scan), batch_(batch), tile_(static_cast<int>(tile)),
-        variant_(static_cast<int>(variant)) {
+        variant_(static_cast<int>(variant)),
+        shape_{static_cast<int>(fwd_waves), static_cast<int>(fwd_views),
+               static_cast<int>(pix_width)} {
     TORCH_CHECK(variant >= -1 && variant <= DD_ADJ_QUAD, "Plan2d: unknown kernel variant");
+    TORCH_CHECK(fwd_waves == 0 || fwd_waves == 1 || fwd_waves == 2 || fwd_waves == 4,
+                "Plan2d: fwd_waves must be 1, 2 or 4 (0 = default)");
+    TORCH_CHECK(fwd_views >= 0 && fwd_views <= 8, "Plan2d: fwd_views must be 1..8 (0 = default)");
+    TORCH_CHECK(pix_width == 0 || pix_width == 16 || pix_width == 32 || pix_width == 64,
+                "Plan2d: pix_width must be 16, 32 or 64 (0 = default)");
     TORCH_CHECK(scan >= DD_PARA && scan <= DD_FAN_ARC, "Plan2d: 2D scan expected");
     TORCH_CHECK(angles.is_cuda() && angles.scalar_type() == torch::kFloat32 &&
                     angles.dim() == 1,
@@ -367,6 +375,16 @@ class Plan2d {
   // Runs op `op` (dd_op) on a [batch, C, ...] input whose B*C equals the
   // planned batch; returns a new output tensor.
   torch::Tensor run(int64_t op, const torch::Tensor& input) {
+    return run_ep(op, input, c10::nullopt, c10::nullopt, 1.0, 0.0);
+  }
+
+  // run() with the fused epilogue  out = alpha * scale * op(input) + beta * c
+  // (scale: an optional one-element device tensor, read when the op runs;
+  // c: an optional tensor of the output's shape).
+  torch::Tensor run_ep(int64_t op, const torch::Tensor& input,
+                       const c10::optional<torch::Tensor>& c,
+                       const c10::optional<torch::Tensor>& scale, double alpha,
+                       double beta) {
     TORCH_CHECK(op >= DD_PROJ && op <= DD_BPROJ_TW && ws_bytes_[op] > 0,
                 "Plan2d: op not available for this scan");
     check_data(input, angles_, "Plan2d");
@@ -386,16 +404,29 @@ class Plan2d {
                                   input.options())
                    : torch::empty({input.size(0), input.size(1), g_.height, g_.width},
                                   input.options());
-    DD_CHECK(dd2d_run_planned(scan_, static_cast<int>(op), &g_, geo_.data_ptr(),
-                              input.data_ptr<float>(), out.data_ptr<float>(),
-                              static_cast<int>(batch_), static_cast<int>(views_), tile_,
-                              variant_, ws_[op].data_ptr(), ws_bytes_[op], cur_stream(input)),
+    dd_epilogue ep{nullptr, nullptr, static_cast<float>(alpha), static_cast<float>(beta)};
+    if (c.has_value()) {
+      check_data(*c, angles_, "Plan2d epilogue term");
+      TORCH_CHECK(c->sizes() == out.sizes(), "Plan2d: epilogue term must have the output's shape");
+      ep.c = c->data_ptr<float>();
+    }
+    if (scale.has_value()) {
+      check_data(*scale, angles_, "Plan2d epilogue scale");
+      TORCH_CHECK(scale->numel() == 1, "Plan2d: epilogue scale must hold one value");
+      ep.scale = scale->data_ptr<float>();
+    }
+    DD_CHECK(dd2d_run_planned_ep(scan_, static_cast<int>(op), &g_, geo_.data_ptr(),
+                                 input.data_ptr<float>(), out.data_ptr<float>(),
+                                 static_cast<int>(batch_), static_cast<int>(views_), tile_,
+                                 variant_, &shape_, &ep, ws_[op].data_ptr(), ws_bytes_[op],
+                                 cur_stream(input)),
              "Plan2d run");
     return out;
   }
 
   int64_t batch() const { return batch_; }
   int64_t tile() const { return tile_; }
+  const dd_shape& shape() const { return shape_; }
   int64_t variant() const { return variant_; }
   int64_t views() const { return views_; }
 
@@ -404,6 +435,7 @@ class Plan2d {
   int64_t batch_;
   int tile_ = 0;
   int variant_ = -1;
+  dd_shape shape_{0, 0, 0};
   int64_t views_ = 0;
   dd_geom2d g_{};
   torch::Tensor angles_, geo_;
@@ -599,10 +631,19 @@ PYBIND11_MODULE(radontorch, m) {
 #undef REG
   pybind11::class_<Plan2d>(m, "Plan2d")
       .def(pybind11::init<int, const torch::Tensor&, const torch::Tensor&, int64_t, int64_t,
-                          int64_t>(),
+                          int64_t, int64_t, int64_t, int64_t>(),
            pybind11::arg("scan"), pybind11::arg("options"), pybind11::arg("angles"),
-           pybind11::arg("batch"), pybind11::arg("tile") = 0, pybind11::arg("variant") = -1)
+           pybind11::arg("batch"), pybind11::arg("tile") = 0, pybind11::arg("variant") = -1,
+           pybind11::arg("fwd_waves") = 0, pybind11::arg("fwd_views") = 0,
+           pybind11::arg("pix_width") = 0)
       .def("run", &Plan2d::run)
+      .def("run_ep", &Plan2d::run_ep, pybind11::arg("op"), pybind11::arg("input"),
+           pybind11::arg("c") = pybind11::none(), pybind11::arg("scale") = pybind11::none(),
+           pybind11::arg("alpha") = 1.0, pybind11::arg("beta") = 0.0)
+      .def_property_readonly("shape", [](const Plan2d& p) {
+        const dd_shape& sh = p.shape();
+        return pybind11::make_tuple(sh.fwd_waves, sh.fwd_views, sh.pix_width);
+      })
       .def_property_readonly("tile", &Plan2d::tile)
       .def_property_readonly("variant", &Plan2d::variant)
       .def_property_readonly("batch", &Plan2d::batch)
@@ -631,10 +672,13 @@ PYBIND11_MODULE(radontorch, m) {
         pybind11::arg("options"), pybind11::arg("angles"));
   m.def("set_instance_tile", [](int t) { dd2d_set_instance_tile(t); });
   m.def("set_adjoint_variant", [](int v) { dd2d_set_adjoint_variant(v); });
+  m.def("set_compensated3d", [](int mode) { dd3d_set_compensated(mode); },
+        pybind11::arg("mode"));
   m.attr("ADJ_PER_PIXEL") = static_cast<int>(DD_ADJ_PER_PIXEL);
   m.attr("ADJ_TILE16") = static_cast<int>(DD_ADJ_TILE16);
   m.attr("ADJ_TILE8") = static_cast<int>(DD_ADJ_TILE8);
   m.attr("ADJ_QUAD") = static_cast<int>(DD_ADJ_QUAD);
+  m.attr("ADJ_BLOCK_WINDOW") = static_cast<int>(DD_ADJ_BLOCK_WINDOW);
   m.attr("PROJ") = static_cast<int>(DD_PROJ);
   m.attr("PROJ_T") = static_cast<int>(DD_PROJ_T);
   m.attr("BPROJ") = static_cast<int>(DD_BPROJ);

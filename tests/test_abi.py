@@ -87,7 +87,7 @@ def test_3d_workspace_query_and_size_limit():
     assert adj >= 360 * (384 + 2) * 512 * 2 * 4
     big = G3(64, 64, 64, 32768, 32768, 1.0, 1.0, 1.0, 1.0, 1.0, 1000.0, 1500.0, 0, 0, 0, 0, 0)
     assert lib.dd3d_workspace_size(3, 1, ctypes.byref(big), 1, 4) == 0
-    big.num_row = (1 << 30) // 32768 - 3  # (rows + 2) * cols just below 2^30
+    big.num_row = (1 << 29) // 32768 - 3  # (rows + 2) * cols just below 2^29
     assert lib.dd3d_workspace_size(3, 1, ctypes.byref(big), 1, 4) > 0
 
 

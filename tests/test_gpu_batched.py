@@ -144,7 +144,7 @@ def test_uneven_batch_equals_per_instance_ops():
         ys = [torch.rand_like(s) for s in sinos]
         from pridediff_amd import load_radontorch
         rt = load_radontorch()
-        for var in (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD):
+        for var in (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD, rt.ADJ_BLOCK_WINDOW):
             ub.plan.set_variant(var)
             backs = ub.unpack_images(ub.projection_t(ub.pack_sinos(ys)))
             for P, a, y, b in zip(projs, angs, ys, backs):
@@ -206,3 +206,57 @@ def test_uniform_autotune_keeps_results():
     print("autotuned tile", info, flush=True)
     with torch.no_grad():
         assert torch.equal(tuned.projection(x), BatchedProjector2d(P, ang, 6, tile=1).projection(x))
+
+
+@pytest.mark.parametrize("scan", ["para", "flat", "arc"])
+def test_uneven_block_window_paths_bitwise(scan):
+    """The windowed 4 x 4 blocks (DD_ADJ_BLOCK_WINDOW) against the
+    per-instance ops bit for bit, on instances whose bins are much narrower
+    than a pixel (a 4-pixel run meets more bins than the window holds, so
+    the serial tail runs), much wider (few bins per run), and tiny detectors
+    (fewer bins than the window: the serial path)."""
+    from pridediff_amd import load_radontorch
+    from pridediff_amd.batched import UnevenBatch2d
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    rt = load_radontorch()
+    projs, angs, ys = [], [], []
+    gen = torch.Generator(device="cuda").manual_seed(77)
+    for i, (n, d, det) in enumerate(((40, 300, 1.2858 / 4), (52, 25, 1.2858 * 6),
+                                     (36, 5, 1.2858 * 10), (64, 93, 1.2858 * 2))):
+        projs.append(projector2d(n, d, 0.7 * 2, det, 595.0, 1085.6, scan_type=scan).cuda())
+        v = 17 + 5 * i
+        angs.append(torch.from_numpy((np.arange(v) * (2 * np.pi / v) + 0.03).astype(np.float32)).cuda())
+        ys.append(torch.rand(v, d, device="cuda", generator=gen))
+    ub = UnevenBatch2d(projs, angs)
+    ops = ("projection_t", "backprojection") + (() if scan == "para" else ("backprojection_w",))
+    with torch.no_grad():
+        for var in (rt.ADJ_BLOCK_WINDOW, rt.ADJ_QUAD):
+            ub.plan.set_variant(var)
+            for op in ops:
+                outs = ub.unpack_images(getattr(ub, op)(ub.pack_sinos(ys)))
+                for P, a, y, o in zip(projs, angs, ys, outs):
+                    want = getattr(P, op)(y[None, None], a)[0, 0]
+                    assert torch.equal(o, want), (scan, var, op, (o - want).abs().max().item())
+
+
+def test_workgroup_shapes_are_bitwise_equal():
+    """Every forward (waves x views per workgroup) and per-pixel adjoint
+    (pixel-tile width) workgroup shape the autotuner sweeps gives the default
+    shape's results bit for bit, at instance tiles 1, 4 and 16, with the
+    epilogue path too."""
+    from pridediff_amd.autotune import FWD_SHAPES, PIX_WIDTHS
+    from pridediff_amd.batched import BatchedProjector2d
+    from pridediff_amd import load_radontorch
+    rt = load_radontorch()
+    P, ang, x, y = _setup(b=16)
+    for tile in (1, 4, 16):
+        ref = BatchedProjector2d(P, ang, 16, tile=tile, variant=rt.ADJ_PER_PIXEL)
+        with torch.no_grad():
+            a0, t0, w0 = ref.projection(x), ref.projection_t(y), ref.backprojection_w(y)
+            for wv in FWD_SHAPES:
+                for pw in PIX_WIDTHS:
+                    bp = BatchedProjector2d(P, ang, 16, tile=tile, variant=rt.ADJ_PER_PIXEL,
+                                            shape=(wv[0], wv[1], pw))
+                    assert torch.equal(bp.projection(x), a0), (tile, wv, pw)
+                    assert torch.equal(bp.projection_t(y), t0), (tile, wv, pw)
+                    assert torch.equal(bp.backprojection_w(y), w0), (tile, wv, pw)

@@ -182,7 +182,7 @@ def test_uneven_64_small_grid_strides_items(c2):
     adj = ("projection_t", "backprojection", "backprojection_w")
     with torch.no_grad():
         ref = {o: getattr(ub, o)(x).clone() for o in ops}
-        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD)
+        variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD, rt.ADJ_BLOCK_WINDOW)
         ref_adj = {}
         for var in variants:
             ub.plan.set_variant(var)

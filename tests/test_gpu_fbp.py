@@ -101,3 +101,51 @@ def test_fbp3d_matches_fp64_pipeline():
     r = O.rel_l2(got, want)
     print(f"cone-flat FBP vs FP64 pipeline: rel_l2={r:.3e}", flush=True)
     assert r <= 1e-5
+
+
+@pytest.mark.parametrize("scan", ["para", "flat", "arc"])
+def test_fused_fidelity_step_matches_fp64_composition(scan):
+    """x - w FBP(A x - y) with the residual fused into the forward's reduction
+    and the update into the backprojection's write, against the same step
+    composed in FP64 from the oracle's Proj, torch's CPU float64 conv2d and
+    the oracle's BackProjWeighted (model/pridediff.py:56-68); and equal to the
+    unfused planned composition.  Replayed from a graph after the weight w
+    changes on the device."""
+    from pridediff_amd.batched import BatchedProjector2d, GraphStep
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    from pridediff_amd.fbp import PlannedFBP2d, fidelity_step, fidelity_step_fused
+    n, d, v, b = 80, 117, 96, 3
+    det = 1.2 * 4 if scan != "arc" else 1.2858 * 4
+    P = projector2d(n, d, 0.7 * 4, det, 595.0, 1085.6, scan_type=scan).cuda()
+    g = O.Geom2D(n, n, d, 0.7 * 4, 0.7 * 4, det / 1085.6 if scan == "arc" else det, 595.0, 1085.6)
+    ang_np = (np.arange(v) * (2 * np.pi / v)).astype(np.float32)
+    ang = torch.from_numpy(ang_np).cuda()
+    x_np = ellipses2d(n, b, seed=21, scale=0.04)
+    x = torch.from_numpy(x_np)[:, None].cuda()
+    y_np = (O.op2d(scan, "proj", x_np, ang_np, g, "f64") * 1.02
+            + 0.01 * np.random.default_rng(22).standard_normal((b, v, d))).astype(np.float32)
+    y = torch.from_numpy(y_np)[:, None].cuda()
+    BP = BatchedProjector2d(P, ang, b)
+    fbp = PlannedFBP2d(P, ang, b, plan=BP)
+    w = torch.tensor([0.35], device="cuda")
+    with torch.no_grad():
+        fused = fidelity_step_fused(fbp, x, y, w).clone()
+        unfused = fidelity_step(BP.projection, fbp, x, y, w[0]).clone()
+    d_fu = O.rel_l2(fused.cpu().numpy(), unfused.cpu().numpy())
+    r64 = O.op2d(scan, "proj", x_np, ang_np, g, "f64") - y_np
+    want = x_np - 0.35 * _fbp64_2d(P, g, scan, r64, ang_np)
+    r = O.rel_l2(fused[:, 0].cpu().numpy(), want)
+    # the step itself (x - x'), where the filtered backprojection lives
+    rs = O.rel_l2((x - fused)[:, 0].cpu().numpy(), x_np - want)
+    print(f"{scan} fused DC step: vs FP64 {r:.3e} (update part {rs:.3e}), vs unfused {d_fu:.3e}",
+          flush=True)
+    assert r <= 1e-6 and rs <= 1e-5 and d_fu <= 1e-7
+    with torch.no_grad():
+        step = GraphStep(lambda a, c: fidelity_step_fused(fbp, a, c, w), x, y)
+        for i, wv in enumerate((0.35, 0.1, 0.6)):
+            w.fill_(wv)
+            eager = fidelity_step_fused(fbp, x, y, w).clone()
+            got = step.replay(x, y).clone()
+            torch.cuda.synchronize()
+            _ = torch.randn(1000, device="cuda").sum()
+            assert torch.equal(got, eager), f"replay {i} (w = {wv})"

@@ -71,3 +71,30 @@ def test_single_views_function_forward_backward_and_speed():
     assert torch.equal(one, loop_out)
     print(f"{ang.numel()} views: loop {t_loop * 1e3:.2f} ms, one launch {t_one * 1e3:.2f} ms",
           flush=True)
+
+
+@pytest.mark.parametrize("scan", ["para", "flat", "arc"])
+def test_views_match_fp64_single_view_ops(scan):
+    """Each view's image of the one-launch view-resolved backprojection
+    against the FP64 oracle's op on that one view (bprj_sv_fun,
+    vvbp_tensor_net.py:11-24), for the three sinogram -> image ops."""
+    from pridediff_amd.views import backprojection_per_view
+    P, ang, y = _setup(scan, n=64, d=93, v=24, b=2)
+    det = 1.2858 * 6
+    g = O.Geom2D(64, 64, 93, 0.7 * 6, 0.7 * 6, det / 1085.6 if scan == "arc" else det,
+                 595.0, 1085.6)
+    ang_np = ang.cpu().numpy()
+    y_np = y[:, 0].cpu().numpy()
+    ops = {"bprojection": "bproj", "projection_t": "proj_t", "bprojection_w": "bproj_w"}
+    for op, oop in ops.items():
+        if scan == "para" and oop == "bproj_w":
+            oop = "bproj"   # the reference maps para's weighted BP to BackProj (projector.py:49)
+        with torch.no_grad():
+            per = backprojection_per_view(P, y, ang, op).cpu().numpy()
+        worst = 0.0
+        for v in range(0, 24, 5):
+            want = O.op2d(scan, oop, np.ascontiguousarray(y_np[:, v:v + 1]), ang_np[v:v + 1], g, "f64")
+            r = O.rel_l2(per[:, v], want)
+            worst = max(worst, r)
+            assert r <= 1e-5, (scan, op, v, r)
+        print(f"{scan} views2d {op} vs FP64 single-view op: worst rel_l2 {worst:.3e}", flush=True)
