@@ -5,7 +5,8 @@
  * geometry structs, a hipStream_t passed as void*.  Every entry point is
  * stream-ordered and returns a dd_status; none synchronises the host and none
  * allocates memory the caller can see.  Scratch space is passed in
- * (dd*_workspace_size + the `ws` argument).
+ * (dd*_workspace_size + the `ws` argument), except for dd2d_run_noalloc,
+ * which uses a fixed per-device buffer inside the library.
  *
  * Reference interface replaced: the raw-pointer C level of deeprecon,
  *   deeprecon/include/dis2d/{FanArc,FanFlat,Para}DisCUDA2d.h and
@@ -30,7 +31,9 @@ typedef enum {
   DD_UNSUPPORTED = 2,  /* op/geometry pair the reference does not define */
   DD_LAUNCH_FAIL = 3,  /* a kernel launch or HIP call failed */
   DD_NO_WORKSPACE = 4, /* workspace too small or absent */
-  DD_NO_DEVICE = 5     /* no HIP device visible */
+  DD_NO_DEVICE = 5,    /* no HIP device visible */
+  DD_TOO_LARGE = 6     /* the problem needs more scratch than dd2d_run_noalloc's
+                          fixed buffer (DD_NOALLOC_SCRATCH_BYTES) holds */
 } dd_status;
 
 /* Scan geometries (the reference's scan_type, projector.py:104,273,452). */
@@ -85,6 +88,21 @@ size_t dd2d_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
 dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
                    float* output, const float* angles, int batch, int views,
                    void* ws, size_t ws_bytes, void* stream);
+
+/* dd2d_run without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation).  No counterpart in the reference.  Scratch comes
+ * from a fixed buffer inside the library, one per device, resident until the
+ * process ends; the call allocates and frees nothing.  A problem whose
+ * dd2d_workspace_size() exceeds DD_NOALLOC_SCRATCH_BYTES is rejected with
+ * DD_TOO_LARGE (a 512x512 fan-arc slice with 720 views and 736 bins needs
+ * 27 MB at batch 1).  Results equal dd2d_run's bit for bit.  Every call on
+ * one device shares that device's buffer: calls that overlap on one device
+ * (two streams, two host threads) race and must be ordered by the caller. */
+#define DD_NOALLOC_SCRATCH_BYTES (64u << 20)
+dd_status dd2d_run_noalloc(int scan, int op, const dd_geom2d* g, const float* input,
+                           float* output, const float* angles, int batch, int views,
+                           void* stream);
 
 /* View-resolved sinogram -> image op (DD_PROJ_T, DD_BPROJ or DD_BPROJ_W):
  * output [batch][views][H][W] holds each view's contribution alone, equal

@@ -21,6 +21,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 
 #include "dd.h"
 #include "dd2d_body.h"
@@ -576,6 +577,33 @@ dd_status launch_views(const Geo2& g, int op, const float* input, float* output,
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
 }
 
+// Fixed scratch of dd2d_run_noalloc: one copy per device, resident until the
+// process ends.  Host code never uses the symbol's name as an address (it is a
+// host-side stand-in); it uses the per-device address that
+// hipGetSymbolAddress returns, taken once per device with that device current.
+__device__ __align__(256) char scratch[DD_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (nullptr on failure).
+void* scratch_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (p) return p;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+  if (!p) {
+    void* q = nullptr;
+    if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return nullptr;
+    g_scratch_ptr[dev].store(q, std::memory_order_release);
+    p = q;
+  }
+  return p;
+}
+
 dd_status check_op(int scan, int op) {
   if (scan < DD_PARA || scan > DD_FAN_ARC) return DD_BAD_ARG;
   if (op < DD_PROJ || op > DD_BPROJ_TW) return DD_BAD_ARG;
@@ -620,6 +648,26 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
+  launch_setup(q, angles, w, L, st);
+  return run_tiles(q, op, adjoint_variant(-1, q), input, output, geo_ptrs(w, L),
+                   w + L.geo_total, L, st);
+}
+
+dd_status dd2d_run_noalloc(int scan, int op, const dd_geom2d* g, const float* input,
+                           float* output, const float* angles, int batch, int views,
+                           void* stream) {
+  dd_status rc = check_op(scan, op);
+  if (rc != DD_OK) return rc;
+  if (!input || !output || !angles) return DD_BAD_ARG;
+  Geo2 q;
+  if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
+  const Layout L = make_layout(q, op);
+  if (L.geo_total + L.data_total > DD_NOALLOC_SCRATCH_BYTES) return DD_TOO_LARGE;
+  char* w = static_cast<char*>(scratch_ptr());
+  if (!w) return DD_LAUNCH_FAIL;
+  // every table and buffer in the scratch is fully written by this call's
+  // own launches before it is read, so nothing needs clearing between calls
+  auto st = static_cast<hipStream_t>(stream);
   launch_setup(q, angles, w, L, st);
   return run_tiles(q, op, adjoint_variant(-1, q), input, output, geo_ptrs(w, L),
                    w + L.geo_total, L, st);
