@@ -55,6 +55,9 @@ def parse():
                          "for profiling runs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--counters-from", metavar="DIR",
+                    help="summarize rocprofv3 --pmc outputs under DIR into "
+                         "profiles/kernel_counters.json and exit (no GPU needed)")
     ap.add_argument("--workload", default="fanarc",
                     choices=["fanarc", "uneven", "cone_flat", "helical", "fidelity"],
                     help="fanarc: 512^2 fan-arc batch 16 (headline); cone_flat: 256^3 cone-flat "
@@ -101,7 +104,7 @@ def roofline_for(key, kernel, ms, b_alg, step_ms=None, n_ops=2):
     ``achieved`` = the bytes the kernel moves through that path per launch
     (rocprofv3 counters of the same launch, committed beside this script) /
     the launch time measured here with HIP events; ``peak`` = CUs x 64 B/clk
-    x the measured shader clock.  HBM is reported as the real fraction of
+    x the 2.4 GHz peak engine clock.  HBM is reported as the real fraction of
     8 TB/s (FETCH_SIZE + WRITE_SIZE per launch / time), and ``effective_GBps``
     is the algorithmic view-pass bytes / time (it exceeds the HBM peak
     because caches serve the per-view re-reads).
@@ -125,6 +128,104 @@ def roofline_for(key, kernel, ms, b_alg, step_ms=None, n_ops=2):
         out["counts_from"] = ("rocprofv3 --pmc runs of this workload, "
                               "profiles/kernel_counters.md; times measured in this run")
     return out
+
+
+# Counter groups per workload: key -> kernels whose dispatches make up one
+# HIP-event-timed launch of that key (a forward is gather + chunk reduction).
+COUNTER_KEYS = {
+    "fanarc": {"fanarc_b16:k_pixels<16,0>": ["k_pixels<16, 0, "],
+               "fanarc_b16:k_bins_split<16,0>": ["k_bins_split<16, 0>", "k_bins_reduce<0>"],
+               "fanarc_plain:k_pixels<1,0>": ["k_pixels<1, 0, "],
+               "fanarc_plain:k_bins_split<1,0>": ["k_bins_split<1, 0>", "k_bins_reduce<0>"]},
+    "uneven": {"uneven64:k_batch_quads_window": ["k_batch_quads<0, 8>"],
+               "uneven64:k_batch_bins": ["k_batch_bins<0>", "k_batch_reduce<0>"]},
+    "cone_flat": {"cone_flat:k_voxels3d": ["k_voxels3d<2, 0, true>"],
+                  "cone_flat:k_bins3d": ["k_bins3d<2, 0>"]},
+    "helical": {"helical:k_voxels3d": ["k_voxels3d<4, 0, false>"],
+                "helical:k_bins3d": ["k_bins3d<4, 0>"]},
+}
+
+
+def summarize_counters(base, out_path):
+    """Write the per-kernel counter file ``roofline_for`` reads.
+
+    ``base/<workload>_<pass>/run_counter_collection.csv`` are rocprofv3
+    ``--pmc`` outputs of ``bench.py --workload <workload>`` for the passes
+    A (TA/GRBM/SQ), B (TCP/TD), FETCH (FETCH_SIZE) and WRITE (WRITE_SIZE);
+    the fanarc FETCH/WRITE passes add ``--copy-probe`` (a 512 MiB device
+    copy), whose counters calibrate FETCH_SIZE and WRITE_SIZE to bytes.
+    Values are means over the dispatches of each kernel.
+    """
+    import collections
+    import csv
+
+    def load(path):
+        vals = collections.defaultdict(lambda: collections.defaultdict(list))
+        dur = collections.defaultdict(dict)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"].split("(dd::")[0].split("(char")[0]
+                name = name.replace("void ", "").replace("dd::(anonymous namespace)::", "")
+                vals[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+                dur[name][r["Dispatch_Id"]] = (int(r["End_Timestamp"])
+                                               - int(r["Start_Timestamp"])) * 1e-9
+        return vals, dur
+
+    def find(vals, pat):
+        ks = [k for k in vals if k.startswith(pat)]
+        if len(ks) != 1:
+            raise SystemExit(f"counter summary: {pat!r} matches {ks}")
+        return ks[0]
+
+    def mean(vals, k, c):
+        return statistics.mean(vals[k][c])
+
+    cus, clk = 256, 2.4e9
+    out = {"cus": cus, "l1_bytes_per_clk": 64, "clock_hz": clk, "kernels": {}}
+    F, _ = load(os.path.join(base, "fanarc_FETCH", "run_counter_collection.csv"))
+    W, _ = load(os.path.join(base, "fanarc_WRITE", "run_counter_collection.csv"))
+    cf, cw = find(F, "__amd_rocclr_copyBuffer"), find(W, "__amd_rocclr_copyBuffer")
+    copy_kib = 512 * 1024
+    fmax, wmax = max(F[cf]["FETCH_SIZE"]), max(W[cw]["WRITE_SIZE"])
+    cal = {"copy_kib": copy_kib, "fetch_kib": fmax, "write_kib": wmax,
+           "fetch_scale": copy_kib / fmax, "write_scale": copy_kib / wmax}
+    for wl, keys in COUNTER_KEYS.items():
+        runs = {p: load(os.path.join(base, f"{wl}_{p}", "run_counter_collection.csv"))
+                for p in ("A", "B", "FETCH", "WRITE")}
+        for key, pats in keys.items():
+            e = {"kernels": [], "pmc_ms": 0.0, "vmem_instructions": 0.0, "waves": 0.0,
+                 "valu_instructions": 0.0, "vmem_bytes": 0.0, "l1_sector_bytes": 0.0,
+                 "fetch_kib_raw": 0.0, "write_kib_raw": 0.0, "_cyc": 0.0, "_ta": 0.0, "_td": 0.0}
+            for pat in pats:
+                (A, tA), (Bv, _), (Fv, _), (Wv, _) = (runs[p] for p in ("A", "B", "FETCH", "WRITE"))
+                ka, kb, kf, kw = find(A, pat), find(Bv, pat), find(Fv, pat), find(Wv, pat)
+                e["kernels"].append(ka)
+                e["pmc_ms"] += statistics.mean(tA[ka].values()) * 1e3
+                e["_cyc"] += mean(A, ka, "GRBM_GUI_ACTIVE") / 8   # summed over 8 XCDs
+                e["_ta"] += mean(A, ka, "TA_TA_BUSY_sum") / cus
+                e["_td"] += mean(Bv, kb, "TD_TD_BUSY_sum") / cus
+                e["vmem_instructions"] += mean(A, ka, "SQ_INSTS_VMEM_RD")
+                e["valu_instructions"] += mean(A, ka, "SQ_INSTS_VALU")
+                e["waves"] += mean(A, ka, "SQ_WAVES")
+                e["vmem_bytes"] += mean(A, ka, "TA_TA_BUSY_sum") * out["l1_bytes_per_clk"]
+                e["l1_sector_bytes"] += mean(Bv, kb, "TCP_TOTAL_CACHE_ACCESSES_sum") * 32
+                e["fetch_kib_raw"] += mean(Fv, kf, "FETCH_SIZE")
+                e["write_kib_raw"] += mean(Wv, kw, "WRITE_SIZE")
+            cyc, ta, td = e.pop("_cyc"), e.pop("_ta"), e.pop("_td")
+            e["measured_clock_hz"] = cyc / (e["pmc_ms"] * 1e-3)
+            e["ta_busy_frac"], e["td_busy_frac"] = ta / cyc, td / cyc
+            e["hbm_bytes"] = (e["fetch_kib_raw"] * cal["fetch_scale"]
+                              + e["write_kib_raw"] * cal["write_scale"]) * 1024
+            out["kernels"][key] = e
+    out["calibration"] = cal
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    for k, v in out["kernels"].items():
+        t = v["pmc_ms"] * 1e-3
+        print(f"{k:34s} {v['pmc_ms']:8.3f} ms  clk {v['measured_clock_hz'] / 1e9:.2f} GHz  "
+              f"TA {v['ta_busy_frac']:.2f}  vmem {v['vmem_bytes'] / t / (cus * 64 * clk):.2f}  "
+              f"HBM {v['hbm_bytes'] / 1e9:7.3f} GB = {v['hbm_bytes'] / t / HBM_PEAK:.3f} of peak")
+    print(json.dumps(cal))
 
 
 def roofline_c1(kernel, ms, b_alg, step_s):
@@ -586,6 +687,9 @@ def c2_cpu_sample(shapes_sel):
 
 def main():
     args = parse()
+    if args.counters_from:
+        summarize_counters(args.counters_from, COUNTERS)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
                           "n_gpus": args.gpus}))
