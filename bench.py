@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner sweep (automatic tile, default kernel variant); "
                          "for profiling runs")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="uneven workload: run this kernel variant (DD_ADJ_* | DD_FWD_PREFIX) "
+                         "at one workgroup per item instead of tuning (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--counters-from", metavar="DIR",
@@ -138,7 +141,12 @@ COUNTER_KEYS = {
                "fanarc_plain:k_pixels<1,0>": ["k_pixels<1, 0, "],
                "fanarc_plain:k_bins_split<1,0>": ["k_bins_split<1, 0>", "k_bins_reduce<0>"]},
     "uneven": {"uneven64:k_batch_quads_window": ["k_batch_quads<0, 8>"],
-               "uneven64:k_batch_bins": ["k_batch_bins<0>", "k_batch_reduce<0>"]},
+               "uneven64:k_batch_quads": ["k_batch_quads<0, 0>"],
+               "uneven64:k_batch_quads_prefix": ["k_batch_prefix_sino<true>",
+                                                 "k_batch_quads_prefix<0>"],
+               "uneven64:k_batch_bins": ["k_batch_bins<0, false>", "k_batch_reduce<0>"],
+               "uneven64:k_batch_bins_prefix": ["k_batch_prefix", "k_batch_bins<0, true>",
+                                                "k_batch_reduce<0>"]},
     "cone_flat": {"cone_flat:k_voxels3d": ["k_voxels3d<2, 0, true>"],
                   "cone_flat:k_bins3d": ["k_bins3d<2, 0>"]},
     "helical": {"helical:k_voxels3d": ["k_voxels3d<4, 0, false>"],
@@ -173,9 +181,9 @@ def summarize_counters(base, out_path):
 
     def find(vals, pat):
         ks = [k for k in vals if k.startswith(pat)]
-        if len(ks) != 1:
+        if len(ks) > 1:
             raise SystemExit(f"counter summary: {pat!r} matches {ks}")
-        return ks[0]
+        return ks[0] if ks else None
 
     def mean(vals, k, c):
         return statistics.mean(vals[k][c])
@@ -193,6 +201,8 @@ def summarize_counters(base, out_path):
         runs = {p: load(os.path.join(base, f"{wl}_{p}", "run_counter_collection.csv"))
                 for p in ("A", "B", "FETCH", "WRITE")}
         for key, pats in keys.items():
+            if any(find(runs[p][0], pat) is None for pat in pats for p in runs):
+                continue  # a kernel this run's configuration does not launch
             e = {"kernels": [], "pmc_ms": 0.0, "vmem_instructions": 0.0, "waves": 0.0,
                  "valu_instructions": 0.0, "vmem_bytes": 0.0, "l1_sector_bytes": 0.0,
                  "fetch_kib_raw": 0.0, "write_kib_raw": 0.0, "_cyc": 0.0, "_ta": 0.0, "_td": 0.0}
@@ -243,7 +253,7 @@ def timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True, cpu=False):
+def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True, cpu=False, variant=None):
     """64 fan-arc instances, N in {256,384,512}, V in [96,720], D = ceil(1.44 N):
     one launch per op and phase inside one HIP graph vs 64 plain calls."""
     from pridediff_amd.autotune import tune_uneven
@@ -264,7 +274,10 @@ def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True, cpu=False):
         imgs.append(torch.from_numpy(ellipses2d(n, 1, seed=3000 + i, scale=0.04))[0].to(dev))
     ub = UnevenBatch2d(projs, angs)
     tune = {}
-    if tune_grid:
+    if variant is not None:
+        ub.plan.set_variant(variant)
+        tune = {"grid": 0, "variant": variant}
+    elif tune_grid:
         tune_uneven(ub, report=tune)
     x = ub.pack_images(imgs)
     with torch.no_grad():
@@ -297,10 +310,13 @@ def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True, cpu=False):
            "algorithmic_GB": b_alg / 1e9, "grid": tune.get("grid"),
            "variant": tune.get("variant"),
            "forward_ms": fp_ms / max(fp_n, 1), "adjoint_ms": bp_ms / max(bp_n, 1)}
+    var = tune.get("variant", rt.ADJ_QUAD)
+    out["forward_kernel"] = "running sums" if var & rt.FWD_PREFIX else "pixel walk"
     adj_name = {rt.ADJ_QUAD: "k_batch_quads", rt.ADJ_BLOCK_WINDOW: "k_batch_quads_window",
-                rt.ADJ_TILE16: "k_batch_tile"}.get(tune.get("variant", rt.ADJ_QUAD),
-                                                   "k_batch_pixels")
-    dom = (("k_batch_bins", out["forward_ms"]) if out["forward_ms"] >= out["adjoint_ms"]
+                rt.ADJ_BLOCK_PREFIX: "k_batch_quads_prefix",
+                rt.ADJ_TILE16: "k_batch_tile"}.get(var & ~rt.FWD_PREFIX, "k_batch_pixels")
+    fwd_name = "k_batch_bins_prefix" if var & rt.FWD_PREFIX else "k_batch_bins"
+    dom = ((fwd_name, out["forward_ms"]) if out["forward_ms"] >= out["adjoint_ms"]
            else (adj_name, out["adjoint_ms"]))
     out["roofline"] = roofline_for("uneven64:" + dom[0], dom[0], dom[1], b_alg / 2)
     if check:
@@ -699,7 +715,7 @@ def main():
     if args.workload == "uneven":
         dev = torch.device("cuda:0")
         c2 = uneven_c2(torch, dev, args.steps, args.warmup, not args.no_accuracy, not args.no_tune,
-                       cpu=not args.no_cpu)
+                       cpu=not args.no_cpu, variant=args.variant)
         samples = c2.pop("detector_samples")
         print(json.dumps({
             "metric": "FP+BP detector samples/s & %HBM roofline, 64 uneven fan-arc instances",

@@ -152,6 +152,21 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
 /* Uneven batches only: the 4 x 4 pixel blocks, each line's bins read as one
  * window of loads instead of one bin at a time. */
 #define DD_ADJ_BLOCK_WINDOW 4
+/* Uneven batches only: the 4 x 4 pixel blocks through per-view running
+ * integrals of the sinogram along the detector (FP64 knots at the bin
+ * edges): each pixel's value from a view is the difference of the running
+ * integral at its two edges, found by one merge walk per 4-pixel run.
+ * PROJ_T and BPROJ only (BPROJ_W, whose weight varies per pixel, runs
+ * DD_ADJ_BLOCK_WINDOW).  Same operator, different rounding: not bit-equal
+ * to the other variants. */
+#define DD_ADJ_BLOCK_PREFIX 5
+/* Flag OR-ed into a variant, uneven batches only: the image -> sinogram ops
+ * PROJ and BPROJ_T integrate each bin over each line as a difference of the
+ * line's running pixel sums (accumulated in FP64, stored as FP32 hi/lo
+ * pairs) instead of walking the pixels the bin covers.  Same operator,
+ * different rounding (not bit-equal to the walk; a non-finite pixel then
+ * reaches every later bin of its line).  BPROJ_TW keeps the walk. */
+#define DD_FWD_PREFIX 0x100
 
 /* `tile` = instances sharing one geometry evaluation (1, 2, 4, 8 or 16;
  * 0 = automatic); it must be the same in both calls.  `variant` selects the
@@ -234,9 +249,10 @@ size_t dd2d_batch_device_bytes(const dd2d_batch* b);
 /* Grid size in workgroups, each striding over the work items (0 = one
  * workgroup per work item). */
 void dd2d_batch_set_grid(dd2d_batch* b, int workgroups);
-/* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL, DD_ADJ_TILE16 or
+/* Sinogram -> image kernel of the batch: DD_ADJ_PER_PIXEL, DD_ADJ_TILE16,
  * DD_ADJ_QUAD (in a batch: one thread per 4x4 pixel block, four lines of
- * four pixels; the default). */
+ * four pixels; the default), DD_ADJ_BLOCK_WINDOW or DD_ADJ_BLOCK_PREFIX,
+ * optionally OR-ed with DD_FWD_PREFIX for the image -> sinogram ops. */
 void dd2d_batch_set_variant(dd2d_batch* b, int variant);
 dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes,
                            void* stream);

@@ -29,7 +29,7 @@ CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
 
 
-SWEEP_VERSION = 7  # bumped when the set of swept parameters changes
+SWEEP_VERSION = 8  # bumped when the set of swept parameters changes
 
 _LIB_DIGEST = None
 
@@ -159,14 +159,21 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
 
 
 def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(0, 2, 8, 32, 128, 512), variants=None,
-                use_cache: bool = True, report: dict | None = None) -> UnevenBatch2d:
+                use_cache: bool = True, report: dict | None = None,
+                prefix: bool = True) -> UnevenBatch2d:
     """Sets the fastest grid size and kernel variant on ``batch``
-    (in place).  Grid 0 is one workgroup per work item (the default)."""
+    (in place).  Grid 0 is one workgroup per work item (the default).
+    ``prefix``: also sweep the running-sum adjoint (``ADJ_BLOCK_PREFIX``)
+    and try the winner with the running-sum forward (``FWD_PREFIX``); both
+    round differently from the walks (same operator, errors of the same
+    size against FP64)."""
     if variants is None:
         variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_QUAD, rt.ADJ_BLOCK_WINDOW)
+        if prefix:
+            variants += (rt.ADJ_BLOCK_PREFIX,)
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
     sig = sorted(batch.shapes)
-    key = f"{_device_key()}|uneven|{batch.scan_type}|{sig}"
+    key = f"{_device_key()}|uneven|{batch.scan_type}|{sig}|prefix{int(prefix)}"
     table = _load() if use_cache else {}
     if key in table:
         batch.plan.set_grid(table[key]["grid"])
@@ -186,6 +193,16 @@ def tune_uneven(batch: UnevenBatch2d, grids_per_cu=(0, 2, 8, 32, 128, 512), vari
                 results[(gpc * cus, var)] = time_ms(lambda: (batch.plan.run(rt.PROJ, x),
                                                              batch.plan.run(rt.PROJ_T, y)))
     best = min(results, key=results.get)
+    if prefix:
+        # the running-sum forward on the winner (not bit-equal to the walk:
+        # same operator, its own rounding)
+        var = best[1] | rt.FWD_PREFIX
+        batch.plan.set_grid(best[0])
+        batch.plan.set_variant(var)
+        with torch.no_grad():
+            results[(best[0], var)] = time_ms(lambda: (batch.plan.run(rt.PROJ, x),
+                                                       batch.plan.run(rt.PROJ_T, y)))
+        best = min(results, key=results.get)
     batch.plan.set_grid(best[0])
     batch.plan.set_variant(best[1])
     entry = {"grid": best[0], "variant": best[1], "ms": results[best],

@@ -334,12 +334,12 @@ __device__ __forceinline__ void bins_accumulate(
   if (l0 >= l1) return;
   // the next line's geometry is loaded while this line is worked on
   LineGeo lg_next = lv[l0];
-  float lw_next = (WM == W_FP) ? pv[l0] : kbp;
+  float lw_next = (WM == W_FP) ? const_load(pv + l0) : kbp;
   for (int l = l0; l < l1; ++l) {
     const LineGeo lg = lg_next;
     const float lw = lw_next;
     lg_next = lv[min(l + 1, l1 - 1)];
-    if constexpr (WM == W_FP) lw_next = pv[min(l + 1, l1 - 1)];
+    if constexpr (WM == W_FP) lw_next = const_load(pv + min(l + 1, l1 - 1));
     const EdgePos p1 = edge_pos(e1, lg.a0, lg.inv_p, N);
     const EdgePos p2 = edge_from_next_lane(p1);
     const float* __restrict__ line = src + static_cast<size_t>(l) * N *
@@ -424,6 +424,92 @@ __device__ __forceinline__ void bins_accumulate(
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// 3a'. image -> sinogram through running sums along the lines (one instance
+// per geometry, no per-pixel weight)
+// ---------------------------------------------------------------------------
+// A bin's integral over a line is F(t2) - F(t1), F the running sum of the
+// line's pixels, linear inside each pixel: F(t) = C_c + x_c (t - c),
+// C_c = x_0 + ... + x_{c-1}, t in pixel units clamped to [0, n].  Each
+// (bin, line) is then one lookup instead of a walk over the 1-3 pixels the
+// bin covers.  C is accumulated in FP64 and stored as two floats, hi = fl(C)
+// and lo = fl(C - hi), beside the pixel itself, {hi_c, lo_c, x_c, 0} (one
+// aligned 16-byte load), so differences of nearby sums lose nothing to
+// cancellation: F(t2) - F(t1) = (hi2 - hi1) + (m2 - m1) with m = lo + x f,
+// where hi2 - hi1 rounds once, relative to the result.  A line of n pixels
+// holds n + 1 entries (entry n: the total, x = 0).
+__host__ __device__ constexpr int prefix_entries(int n) { return n + 1; }
+
+// Running sums of one line of n pixels x[0], x[stride], ... into
+// out[0 .. n], by one workgroup of blockDim.x threads (a power of two
+// <= 1024), each owning a contiguous segment.
+__device__ __forceinline__ void prefix_line(const float* __restrict__ x, long long stride,
+                                            int n, float4* __restrict__ out,
+                                            double* __restrict__ lds) {
+  const int t = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+  const int seg = (n + nt - 1) / nt;
+  const int p0 = min(t * seg, n), p1 = min(p0 + seg, n);
+  double s = 0.0;
+  for (int p = p0; p < p1; ++p) s += static_cast<double>(x[p * stride]);
+  lds[t] = s;
+  __syncthreads();
+  // inclusive scan of the segment sums (Hillis-Steele)
+  for (int d = 1; d < nt; d *= 2) {
+    const double add = (t >= d) ? lds[t - d] : 0.0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  double c = lds[t] - s;  // sum of the segments before this one
+  auto entry = [](double v, float xv) {
+    const float hi = static_cast<float>(v);
+    return make_float4(hi, static_cast<float>(v - static_cast<double>(hi)), xv, 0.0f);
+  };
+  for (int p = p0; p < p1; ++p) {
+    const float xv = x[p * stride];
+    out[p] = entry(c, xv);
+    c += static_cast<double>(xv);
+  }
+  if (t == nt - 1) out[n] = entry(lds[nt - 1], 0.0f);
+  __syncthreads();  // lds is reused by the next line
+}
+
+// Sum over lines [l0, l1) of one bin's integrals over the lines (no bin
+// weight), from the lines' running-sum tables tab[line][n + 1]; e1 is the
+// bin's lower edge, the upper edge's sum comes from the next lane.  The whole
+// wavefront calls this together (the DPP reads the next lane); lane 63's sum
+// is meaningless and not written.
+template <int WM>
+__device__ __forceinline__ float bins_accumulate_prefix(
+    double e1, const LineGeo* __restrict__ lv, const float* __restrict__ pv,
+    const float4* __restrict__ tab, float kbp, int l0, int l1, int n) {
+  float acc = 0.0f;
+  if (l0 >= l1) return acc;
+  LineGeo lg_next = lv[l0];
+  float lw_next = (WM == W_FP) ? const_load(pv + l0) : kbp;
+  const float4* __restrict__ row = tab + static_cast<size_t>(l0) * prefix_entries(n);
+  for (int l = l0; l < l1; ++l, row += prefix_entries(n)) {
+    const LineGeo lg = lg_next;
+    const float lw = lw_next;
+    lg_next = lv[min(l + 1, l1 - 1)];
+    if constexpr (WM == W_FP) lw_next = const_load(pv + min(l + 1, l1 - 1));
+    const double t = fmin(fmax((e1 - lg.a0) * lg.inv_p, 0.0), static_cast<double>(n));
+    const double fl = floor(t);
+    const float f = static_cast<float>(t - fl);
+    const float4 e = row[static_cast<int>(fl)];
+    const float m = fmaf(f, e.z, e.y);
+    constexpr int kWaveShl1 = 0x130;  // DPP wave_shl:1: lane i reads lane i + 1
+    const float hn = __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, e.x), kWaveShl1, 0xf, 0xf,
+                                           false));
+    const float mn = __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, m), kWaveShl1, 0xf, 0xf,
+                                           false));
+    acc = fmaf((hn - e.x) + (mn - m), lw, acc);
+  }
+  return acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -968,6 +1054,188 @@ __device__ __forceinline__ void blocks_body(
         for (int b = 0; b < BT; ++b)
           if (b0 + b < g.B)
             out[(static_cast<size_t>(b0 + b) * g.H + r0 + dr) * g.W + c0 + dc] = acc[dr][dc][b];
+}
+
+// ---------------------------------------------------------------------------
+// 3b'. sinogram -> image through running sums over the bins (one instance
+// per geometry, no per-pixel weight)
+// ---------------------------------------------------------------------------
+// A pixel's value from one view is (G(i + 1) - G(i)) / P, G the running
+// integral of the walk-ordered sinogram s (FP weight folded in) along the
+// common axis, piecewise linear between the knots (e_k, R_k), R_k =
+// sum_{j<k} s_j (e_{j+1} - e_j), k = 0 .. D.  The knots are FP64 (the running
+// sum is kept as it is accumulated) and a view's table is {e_k, R_k}: one
+// aligned 16-byte load per bin the walk visits, and no per-(bin, pixel)
+// overlap terms at all.
+
+// Knots of view v of one instance's sinogram p [V][D] (detector order) into
+// that view's table tab[0 .. D] (walk order, s scaled by the FP weight when
+// FOLD_W), by one workgroup of blockDim.x threads (a power of two <= 1024).
+template <bool FOLD_W>
+__device__ __forceinline__ void prefix_sino_view(
+    const Geo2& g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const float* __restrict__ wfp, const float* __restrict__ p, int v, double2* __restrict__ tab,
+    double* __restrict__ lds) {
+  const int t = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+  const int D = g.D;
+  const int flip = views[v].flip;
+  const double* __restrict__ ev = edges + static_cast<size_t>(v) * (D + 1);
+  const float* __restrict__ pv = p + static_cast<size_t>(v) * D;
+  auto term = [&](int k) {
+    const float wk = FOLD_W ? wfp[static_cast<size_t>(v) * D + k] : 1.0f;
+    return static_cast<double>(pv[flip ? D - 1 - k : k] * wk) * (ev[k + 1] - ev[k]);
+  };
+  const int seg = (D + nt - 1) / nt;
+  const int k0 = min(t * seg, D), k1 = min(k0 + seg, D);
+  double s = 0.0;
+  for (int k = k0; k < k1; ++k) s += term(k);
+  lds[t] = s;
+  __syncthreads();
+  for (int d = 1; d < nt; d *= 2) {
+    const double add = (t >= d) ? lds[t - d] : 0.0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  double c = lds[t] - s;
+  for (int k = k0; k < k1; ++k) {
+    tab[k] = make_double2(ev[k], c);
+    c += term(k);
+  }
+  if (t == nt - 1) tab[D] = make_double2(ev[D], lds[nt - 1]);
+  __syncthreads();
+}
+
+// Adds view v's values of the pixels ib + pf .. ib + plst of one line to
+// acc[pf .. plst] (times gw, the line's weight): one first-bin search, then
+// the run's pixel edges and the bin edges merged in one monotone walk, G at
+// each pixel edge interpolated between the two knots around it.  Knot k is
+// kn[k - kmin]; the walk stays within bins kmin .. kmax - 1, which must hold
+// every bin the run's pixel edges fall in (then the result does not depend
+// on the range).  KP is a global or an LDS pointer to the knots as pairs of
+// doubles {e, R}.  Positions are unclamped FP64 pixel units.
+template <typename KP>
+__device__ __forceinline__ double2 knot_at(KP kn, int i) {
+  return make_double2(kn[2 * i], kn[2 * i + 1]);
+}
+
+template <int NP, typename KP>
+__device__ __forceinline__ void pixel_run_view_prefix(
+    const Geo2& g, const ViewGeo& vg, const LineGeo& lg, float pl, float gw, KP kn, int kmin,
+    int kmax, int ib, int pf, int plst, float (&acc)[NP]) {
+  const int ifirst = ib + pf;
+  const double b = lg.a0 * lg.inv_p;  // t = e / P - a0 / P
+  auto pos = [&](double e) { return fma(e, lg.inv_p, -b); };
+  // the bin whose lower knot lies at or before the run's first pixel edge
+  // (guessed at that edge)
+  int k = min(max(guess_bin_fast(g, vg, lg.a0 + ifirst * static_cast<double>(pl)), kmin),
+              kmax - 1);
+  double2 k1 = knot_at(kn, k - kmin);
+  double t1 = pos(k1.x);
+  while (k > kmin && t1 > ifirst) {
+    --k;
+    k1 = knot_at(kn, k - kmin);
+    t1 = pos(k1.x);
+  }
+  double2 k2 = knot_at(kn, k + 1 - kmin);
+  double t2 = pos(k2.x);
+  float rw = __builtin_amdgcn_rcpf(static_cast<float>(t2 - t1));
+  const float scale = static_cast<float>(lg.inv_p) * gw;
+  double gprev = 0.0;
+#pragma unroll
+  for (int j = 0; j <= NP; ++j) {
+    if (j < pf || j > plst + 1) continue;
+    const double di = static_cast<double>(ib + j);
+    while (k < kmax - 1 && t2 <= di) {
+      ++k;
+      k1 = k2;
+      t1 = t2;
+      k2 = knot_at(kn, k + 1 - kmin);
+      t2 = pos(k2.x);
+      rw = __builtin_amdgcn_rcpf(static_cast<float>(t2 - t1));
+    }
+    // fraction of bin k below the edge, clamped: 0 below the first knot,
+    // 1 past the last
+    const float fr = __builtin_amdgcn_fmed3f(static_cast<float>(di - t1) * rw, 0.0f, 1.0f);
+    const double gv = fma(k2.y - k1.y, static_cast<double>(fr), k1.y);
+    if (j > pf) acc[j - 1] = fmaf(static_cast<float>(gv - gprev), scale, acc[j - 1]);
+    gprev = gv;
+  }
+}
+
+// One view's contribution to the Q x Q pixel block at (r0, c0) (nr x nc of
+// it inside the image) through the view's knots (pixel_run_view_prefix);
+// the block's lines as in blocks_body.  WM is W_FP or W_BP.
+template <int WM, int Q, typename KP>
+__device__ __forceinline__ void block_view_prefix(
+    const Geo2& g, const ViewGeo& vg, const LineGeo* __restrict__ lv,
+    const float* __restrict__ pv, KP kn, int kmin, int kmax, int r0, int c0, int nr, int nc,
+    float (&acc)[Q][Q]) {
+  static_assert(WM != W_BPW, "per-pixel weights need the bin walk");
+  const float kbp = bp_const(g);
+  if (vg.xaxis) {
+#pragma unroll
+    for (int dr = 0; dr < Q; ++dr) {
+      if (dr >= nr) break;
+      const int l = r0 + dr;
+      const float pl = const_load(pv + l);
+      pixel_run_view_prefix<Q>(g, vg, lv[l], pl, (WM == W_FP) ? pl : kbp, kn, kmin, kmax, c0,
+                               0, nc - 1, acc[dr]);
+    }
+  } else {
+    // lines = image columns c0 + dc, pixels read bottom-up: run position p
+    // (pixel H - r0 - Q + p) is row r0 + Q - 1 - p, real when p >= Q - nr
+#pragma unroll
+    for (int dc = 0; dc < Q; ++dc) {
+      if (dc >= nc) break;
+      const int l = c0 + dc;
+      const float pl = const_load(pv + l);
+      float run[Q];
+#pragma unroll
+      for (int p = 0; p < Q; ++p) run[p] = acc[Q - 1 - p][dc];
+      pixel_run_view_prefix<Q>(g, vg, lv[l], pl, (WM == W_FP) ? pl : kbp, kn, kmin, kmax,
+                               g.H - r0 - Q, Q - nr, Q - 1, run);
+#pragma unroll
+      for (int p = 0; p < Q; ++p) acc[Q - 1 - p][dc] = run[p];
+    }
+  }
+}
+
+// Knot range [kb, ke] of view v that holds every pixel edge of the pixel
+// tile [r0, r1) x [c0, c1) (image rows x columns): the tile's axis extent is
+// spanned by its corners (a pixel edge's axis point is monotone along and
+// across the lines), widened by one knot each side against rounding.
+__device__ __forceinline__ int2 knot_range(const Geo2& g, const ViewGeo& vg,
+                                           const double* __restrict__ ev,
+                                           const LineGeo* __restrict__ lv, int r0, int r1,
+                                           int c0, int c1) {
+  double amin = 1e300, amax = -1e300;
+  auto corner = [&](int l, int i) {
+    const LineGeo lg = lv[l];
+    const double a = lg.a0 + i / lg.inv_p;
+    amin = fmin(amin, a);
+    amax = fmax(amax, a);
+  };
+  if (vg.xaxis) {  // lines = rows, pixel edges = columns
+    corner(r0, c0), corner(r0, c1), corner(r1 - 1, c0), corner(r1 - 1, c1);
+  } else {  // lines = columns, pixel edges from the bottom
+    corner(c0, g.H - r1), corner(c0, g.H - r0), corner(c1 - 1, g.H - r1),
+        corner(c1 - 1, g.H - r0);
+  }
+  // last knot <= amin, first knot >= amax (edges ascend in walk order)
+  int lo = 0, hi = g.D;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (ev[mid] <= amin) lo = mid; else hi = mid - 1;
+  }
+  const int kb = max(lo - 1, 0);
+  lo = 0, hi = g.D;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (ev[mid] >= amax) hi = mid; else lo = mid + 1;
+  }
+  const int ke = min(lo + 1, g.D);
+  return make_int2(kb, max(ke, kb + 1));
 }
 
 // sinogram -> image for one pixel, all views.  Each view's search is

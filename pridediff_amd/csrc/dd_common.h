@@ -30,6 +30,16 @@ struct LineGeo {
   double inv_p; // 1 / pixel interval on the axis
 };
 
+// Load of a table the running kernel never writes, through the constant
+// address space: at a wave-uniform address it becomes a scalar load even
+// where the kernel stores floats through pointers the compiler cannot tell
+// apart from the table's (the batch kernels rebuild every pointer from one
+// device block).
+template <typename T>
+__device__ __forceinline__ T const_load(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)(p);
+}
+
 // Integer pixel + fractional position of an axis point on one line.
 // The single definition used by the forward and the adjoint kernels: both
 // see bit-identical overlaps, so the two operators are exact transposes.

@@ -40,15 +40,29 @@ using namespace body2d;
 // into that block.  Kernels take the block's base as an argument and rebuild
 // the pointers from it (BatchInst), so every access is a global-memory one
 // (pointers read from memory would otherwise be generic: flat loads).
-enum Region { kViews, kEdges, kWfp, kLines, kPint, kXrow, kXcol, kSino, kPart, kRegions };
+enum Region {
+  kViews, kEdges, kWfp, kLines, kPint, kXrow, kXcol, kSino, kPart, kPrefRow, kPrefCol,
+  kPrefSino, kKnotRange, kRegions
+};
+
+enum Phase {
+  kLayoutImage = 0,
+  kGatherBins = 1,
+  kLayoutSino = 2,
+  kGatherPixels = 3,
+  kReduceBins = 4,
+  kGatherQuads = 5,   // sinogram -> image with one thread per 4 x 4 pixels
+  kLayoutPrefix = 6,  // running sums of every image row and column
+  kPhases = 7
+};
 
 struct BatchDesc {
   Geo2 g;
   unsigned long long off[kRegions];
   long long img_off;   // floats, into the packed image buffer
   long long sino_off;  // floats, into the packed sinogram buffer
-  int item0[6];        // first work item of each phase (see Phase)
-  int nitems[6];
+  int item0[kPhases];  // first work item of each phase
+  int nitems[kPhases];
 };
 
 // One instance inside a kernel: the descriptor with typed pointers.
@@ -63,18 +77,12 @@ struct BatchInst {
   float* xcol;         // layout scratch: image columns, bottom-up
   float* sino;         // layout scratch: sinogram in walk order
   float* part;         // partial bin sums per line chunk [kSplit][B][V][D]
+  float4* prow;        // running sums of the rows [B][H][W + 1]
+  float4* pcol;        // running sums of the columns, bottom-up [B][W][H + 1]
+  double2* psino;      // running-sum knots over the bins per view [B][V][D + 1]
+  int2* krange;        // knots each 64 x 64 pixel tile needs, per view [V][tiles]
   long long img_off;
   long long sino_off;
-};
-
-enum Phase {
-  kLayoutImage = 0,
-  kGatherBins = 1,
-  kLayoutSino = 2,
-  kGatherPixels = 3,
-  kReduceBins = 4,
-  kGatherQuads = 5,  // sinogram -> image with one thread per 4 x 4 pixels
-  kPhases = 6
 };
 
 constexpr int kBT = 1;  // one geometry per instance
@@ -114,6 +122,10 @@ __device__ __forceinline__ BatchInst load_inst(char* __restrict__ mem, int i) {
   d.xcol = reinterpret_cast<float*>(mem + s.off[kXcol]);
   d.sino = reinterpret_cast<float*>(mem + s.off[kSino]);
   d.part = reinterpret_cast<float*>(mem + s.off[kPart]);
+  d.prow = reinterpret_cast<float4*>(mem + s.off[kPrefRow]);
+  d.pcol = reinterpret_cast<float4*>(mem + s.off[kPrefCol]);
+  d.psino = reinterpret_cast<double2*>(mem + s.off[kPrefSino]);
+  d.krange = reinterpret_cast<int2*>(mem + s.off[kKnotRange]);
   d.img_off = s.img_off;
   d.sino_off = s.sino_off;
   return d;
@@ -142,6 +154,32 @@ __global__ __launch_bounds__(kThreads) void k_batch_setup(
   }
 }
 
+// Per (instance, view): the knot range every 64 x 64 pixel tile of the
+// block items needs (knot_range), for the LDS staging of k_batch_quads_prefix.
+__global__ __launch_bounds__(kThreads) void k_batch_ranges(char* __restrict__ mem, int n_inst,
+                                                           int total) {
+  const BatchDesc* __restrict__ desc = reinterpret_cast<const BatchDesc*>(mem);
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int i = 0, base = 0;
+    while (i < n_inst - 1 && item >= base + desc[i].g.V) {
+      base += desc[i].g.V;
+      ++i;
+    }
+    const BatchInst d = load_inst(mem, i);
+    const int v = item - base;
+    const int tx = (d.g.W + kQuadTile - 1) / kQuadTile;
+    const int tiles = tx * ((d.g.H + kQuadTile - 1) / kQuadTile);
+    const ViewGeo vg = d.views[v];
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
+      const int c0 = (t % tx) * kQuadTile, r0 = (t / tx) * kQuadTile;
+      d.krange[static_cast<size_t>(v) * tiles + t] = knot_range(
+          d.g, vg, d.edges + static_cast<size_t>(v) * (d.g.D + 1),
+          d.lines + static_cast<size_t>(v) * d.g.Lmax, r0, min(r0 + kQuadTile, d.g.H), c0,
+          min(c0 + kQuadTile, d.g.W));
+    }
+  }
+}
+
 template <bool FOLD_W>
 __global__ __launch_bounds__(kThreads) void k_batch_layout(
     char* __restrict__ mem, int n_inst, int phase, int total,
@@ -166,6 +204,28 @@ __global__ __launch_bounds__(kThreads) void k_batch_layout(
   }
 }
 
+// Running-sum items: (line, instance), the H rows first, then the W columns
+// read bottom-up; one workgroup per line.
+__global__ __launch_bounds__(kThreads) void k_batch_prefix(
+    char* __restrict__ mem, int n_inst, int total, const float* __restrict__ in) {
+  __shared__ double lds[kThreads];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(mem, n_inst, kLayoutPrefix, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kLayoutPrefix);
+    const int H = d.g.H, W = d.g.W;
+    const int line = local % (H + W);
+    const int b = local / (H + W);
+    const float* __restrict__ img = in + d.img_off + static_cast<size_t>(b) * H * W;
+    if (line < H)
+      prefix_line(img + static_cast<size_t>(line) * W, 1, W,
+                  d.prow + (static_cast<size_t>(b) * H + line) * prefix_entries(W), lds);
+    else
+      prefix_line(img + static_cast<size_t>(H - 1) * W + (line - H), -static_cast<long long>(W),
+                  H, d.pcol + (static_cast<size_t>(b) * W + line - H) * prefix_entries(H), lds);
+  }
+}
+
 // Gather-bins items: (block of kBinsPerWave bins, view, instance, line
 // chunk), chunk fastest, one wavefront per item (a 64-thread workgroup), so
 // instances whose bin count is not a multiple of 256 keep their lanes busy.
@@ -173,7 +233,9 @@ __global__ __launch_bounds__(kThreads) void k_batch_layout(
 // a grid that is a multiple of 8 the chunk follows the XCD.
 constexpr int kBinsWave = 64;  // threads per gather-bins workgroup
 
-template <int WM>
+// PREFIX: the bins' line integrals come from the running-sum tables
+// (bins_accumulate_prefix) instead of the pixel walk.
+template <int WM, bool PREFIX>
 __global__ __launch_bounds__(kBinsWave) void k_batch_bins(
     char* __restrict__ mem, int n_inst, int total) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -198,10 +260,19 @@ __global__ __launch_bounds__(kBinsWave) void k_batch_bins(
     float acc[kBT];
 #pragma unroll
     for (int b = 0; b < kBT; ++b) acc[b] = 0.0f;
-    bins_accumulate<kBT, WM>(d.g, vg, e1, d.lines + static_cast<size_t>(v) * d.g.Lmax,
-                             d.pint + static_cast<size_t>(v) * d.g.Lmax,
-                             (vg.xaxis ? d.xrow : d.xcol) + b0, bp_const(d.g),
-                             L * chunk / kSplit, L * (chunk + 1) / kSplit, N, acc, edge_lane);
+    if constexpr (PREFIX) {
+      const float4* __restrict__ tab =
+          (vg.xaxis ? d.prow : d.pcol) + static_cast<size_t>(b0) * L * prefix_entries(N);
+      acc[0] = bins_accumulate_prefix<WM>(e1, d.lines + static_cast<size_t>(v) * d.g.Lmax,
+                                          d.pint + static_cast<size_t>(v) * d.g.Lmax, tab,
+                                          bp_const(d.g), L * chunk / kSplit,
+                                          L * (chunk + 1) / kSplit, N);
+    } else {
+      bins_accumulate<kBT, WM>(d.g, vg, e1, d.lines + static_cast<size_t>(v) * d.g.Lmax,
+                               d.pint + static_cast<size_t>(v) * d.g.Lmax,
+                               (vg.xaxis ? d.xrow : d.xcol) + b0, bp_const(d.g),
+                               L * chunk / kSplit, L * (chunk + 1) / kSplit, N, acc, edge_lane);
+    }
     if (edge_lane || k >= d.g.D) continue;
 #pragma unroll
     for (int b = 0; b < kBT; ++b)
@@ -260,6 +331,99 @@ __global__ __launch_bounds__(kThreads) void k_batch_pixels(
   }
 }
 
+// Sinogram running-sum items: the layout items (view, instance), one
+// workgroup per view.
+template <bool FOLD_W>
+__global__ __launch_bounds__(kThreads) void k_batch_prefix_sino(
+    char* __restrict__ mem, int n_inst, int total, const float* __restrict__ in) {
+  __shared__ double lds[kThreads];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(mem, n_inst, kLayoutSino, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kLayoutSino);
+    const int v = local % d.g.V, b = local / d.g.V;
+    prefix_sino_view<FOLD_W>(
+        d.g, d.views, d.edges, d.wfp, in + d.sino_off + static_cast<size_t>(b) * d.g.V * d.g.D,
+        v, d.psino + (static_cast<size_t>(b) * d.g.V + v) * (d.g.D + 1), lds);
+  }
+}
+
+// Block items through the running sums over the bins: per view, the knots the
+// item's 64 x 64 pixel tile needs are staged in LDS (one 16-byte load per
+// thread, double-buffered: view v + 1's load is in flight while view v is
+// walked, one barrier per view), and every 4-pixel run's merge walk reads
+// them there.  A view whose range exceeds the buffer is walked from global
+// memory; both give the same bits (the walk never leaves the range).
+constexpr int kKnotCap = kQuadThreads;  // knots staged per view
+
+template <int WM>
+__global__ __launch_bounds__(kQuadThreads)
+__attribute__((amdgpu_waves_per_eu(4))) void k_batch_quads_prefix(
+    char* __restrict__ mem, int n_inst, int total, float* __restrict__ out) {
+  __shared__ double2 knots[2][kKnotCap];
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int i = find_inst(mem, n_inst, kGatherQuads, item);
+    const BatchInst d = load_inst(mem, i);
+    const int local = item - item0_of(mem, i, kGatherQuads);
+    const int tx = (d.g.W + kQuadTile - 1) / kQuadTile;
+    const int tiles = tx * ((d.g.H + kQuadTile - 1) / kQuadTile);
+    const int t = local % tiles;
+    const int b = local / tiles;
+    const int c0 = (t % tx) * kQuadTile + kBlockQ * (tid % kQuadSide);
+    const int r0 = (t / tx) * kQuadTile + kBlockQ * (tid / kQuadSide);
+    const bool active = c0 < d.g.W && r0 < d.g.H;
+    const int nr = min(kBlockQ, d.g.H - r0), nc = min(kBlockQ, d.g.W - c0);
+    const double2* __restrict__ tab = d.psino + static_cast<size_t>(b) * d.g.V * (d.g.D + 1);
+    const int2* __restrict__ rng = d.krange + t;  // view v at rng[v * tiles]
+    float acc[kBlockQ][kBlockQ];
+#pragma unroll
+    for (int dr = 0; dr < kBlockQ; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < kBlockQ; ++dc) acc[dr][dc] = 0.0f;
+    // stage view 0
+    int2 r = rng[0];
+    bool staged = r.y - r.x + 1 <= kKnotCap;
+    if (staged && tid <= r.y - r.x) knots[0][tid] = tab[r.x + tid];
+    __syncthreads();
+    for (int v = 0; v < d.g.V; ++v) {
+      // the next view's knot for this thread, in flight during this view's walks
+      const int vn = min(v + 1, d.g.V - 1);
+      const int2 rn = rng[static_cast<size_t>(vn) * tiles];
+      const bool staged_n = rn.y - rn.x + 1 <= kKnotCap;
+      double2 kn = make_double2(0.0, 0.0);
+      const bool mine = v + 1 < d.g.V && staged_n && tid <= rn.y - rn.x;
+      if (mine) kn = tab[static_cast<size_t>(vn) * (d.g.D + 1) + rn.x + tid];
+      if (active) {
+        const ViewGeo vg = d.views[v];
+        const LineGeo* __restrict__ lv = d.lines + static_cast<size_t>(v) * d.g.Lmax;
+        const float* __restrict__ pv = d.pint + static_cast<size_t>(v) * d.g.Lmax;
+        if (staged)
+          block_view_prefix<WM, kBlockQ>(
+              d.g, vg, lv, pv, (const __attribute__((address_space(3))) double*)(knots[v & 1]),
+              r.x, r.y, r0, c0, nr, nc, acc);
+        else
+          block_view_prefix<WM, kBlockQ>(
+              d.g, vg, lv, pv,
+              reinterpret_cast<const double*>(tab + static_cast<size_t>(v) * (d.g.D + 1)), 0,
+              d.g.D, r0, c0, nr, nc, acc);
+      }
+      if (mine) knots[(v + 1) & 1][tid] = kn;
+      __syncthreads();
+      r = rn;
+      staged = staged_n;
+    }
+    if (active) {
+      float* __restrict__ o = out + d.img_off + static_cast<size_t>(b) * d.g.H * d.g.W;
+#pragma unroll
+      for (int dr = 0; dr < kBlockQ; ++dr)
+#pragma unroll
+        for (int dc = 0; dc < kBlockQ; ++dc)
+          if (dr < nr && dc < nc) o[static_cast<size_t>(r0 + dr) * d.g.W + c0 + dc] = acc[dr][dc];
+    }
+  }
+}
+
 // Block items: (instance, 64 x 64 pixel tile), one thread per 4 x 4 pixels;
 // KW > 0 reads each line's bins as one window of KW bins.
 template <int WM, int KW>
@@ -293,9 +457,10 @@ struct dd2d_batch {
   std::vector<int> order;        // instance order (heaviest first)
   size_t desc_off = 0, angles_off = 0, bytes = 0;
   size_t total_views = 0;
-  int total[kPhases] = {0, 0, 0, 0, 0, 0};
+  int total[kPhases] = {};
   int grid = 0;                  // workgroups per launch (0 = one per work item)
   int variant = DD_ADJ_QUAD;     // sinogram -> image kernel (per pixel, 16x16 tiles, 4x4 blocks)
+  bool fwd_prefix = false;       // image -> sinogram through running sums (DD_FWD_PREFIX)
   char* dev = nullptr;           // device memory given to setup()
 };
 
@@ -365,7 +530,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
   b->desc_off = take(&off, sizeof(BatchDesc) * n);  // 0: kernels find it at the block base
   b->angles_off = take(&off, sizeof(float*) * n);
   b->inst.resize(n);
-  int acc[kPhases] = {0, 0, 0, 0, 0, 0};
+  int acc[kPhases] = {};
   for (int j = 0; j < n; ++j) {
     const auto& it = items[b->order[j]];
     BatchDesc d{};
@@ -399,6 +564,15 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
     d.off[kSino] = take(&off, sizeof(float) * g.V * g.D * g.Bp);
     d.off[kPart] =
         take(&off, sizeof(float) * static_cast<size_t>(kSplit) * g.Bp * g.V * g.D);
+    d.off[kPrefRow] = take(&off, sizeof(float4) * static_cast<size_t>(g.Bp) * g.H *
+                                     prefix_entries(g.W));
+    d.off[kPrefCol] = take(&off, sizeof(float4) * static_cast<size_t>(g.Bp) * g.W *
+                                     prefix_entries(g.H));
+    d.off[kPrefSino] =
+        take(&off, sizeof(double2) * static_cast<size_t>(g.Bp) * g.V * (g.D + 1));
+    d.off[kKnotRange] = take(&off, sizeof(int2) * static_cast<size_t>(g.V) *
+                                       ((g.W + kQuadTile - 1) / kQuadTile) *
+                                       ((g.H + kQuadTile - 1) / kQuadTile));
     d.img_off = it.image_offset;
     d.sino_off = it.sino_offset;
     const int tiles = it.batch;  // kBT == 1
@@ -409,6 +583,7 @@ dd2d_batch* dd2d_batch_create(const dd2d_batch_item* items, int n) {
         ((g.W + kQuadTile - 1) / kQuadTile) * ((g.H + kQuadTile - 1) / kQuadTile) * tiles;
     d.nitems[kLayoutSino] = g.V * tiles;
     d.nitems[kGatherPixels] = ((g.W + kTile - 1) / kTile) * ((g.H + kTile - 1) / kTile) * tiles;
+    d.nitems[kLayoutPrefix] = (g.H + g.W) * tiles;
     for (int ph = 0; ph < kPhases; ++ph) {
       d.item0[ph] = acc[ph];
       acc[ph] += d.nitems[ph];
@@ -428,11 +603,13 @@ void dd2d_batch_set_grid(dd2d_batch* b, int workgroups) {
 }
 
 void dd2d_batch_set_variant(dd2d_batch* b, int variant) {
-  if (b)
-    b->variant = (variant == DD_ADJ_TILE16 || variant == DD_ADJ_QUAD ||
-                  variant == DD_ADJ_BLOCK_WINDOW)
-                     ? variant
-                     : DD_ADJ_PER_PIXEL;
+  if (!b) return;
+  const int adj = variant & ~DD_FWD_PREFIX;
+  b->variant = (adj == DD_ADJ_TILE16 || adj == DD_ADJ_QUAD || adj == DD_ADJ_BLOCK_WINDOW ||
+                adj == DD_ADJ_BLOCK_PREFIX)
+                   ? adj
+                   : DD_ADJ_PER_PIXEL;
+  b->fwd_prefix = (variant & DD_FWD_PREFIX) != 0;
 }
 
 dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* stream) {
@@ -450,6 +627,8 @@ dd_status dd2d_batch_setup(dd2d_batch* b, void* device_mem, size_t bytes, void* 
   const int total = static_cast<int>(b->total_views);
   hipLaunchKernelGGL(k_batch_setup, dim3(grid_for(b, total)), dim3(kThreads), 0, st, b->dev,
                      reinterpret_cast<const float* const*>(b->dev + b->angles_off), n, total);
+  hipLaunchKernelGGL(k_batch_ranges, dim3(grid_for(b, total)), dim3(kThreads), 0, st, b->dev, n,
+                     total);
   if (hipGetLastError() != hipSuccess) return DD_LAUNCH_FAIL;
   // the host staging vectors die with this call: wait for the uploads
   return hipStreamSynchronize(st) == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
@@ -466,30 +645,68 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
   char* inst = b->dev;  // descriptor table at the block base
   const bool sino_out = (op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW);
   if (sino_out) {
-    hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutImage])),
-                       dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutImage),
-                       b->total[kLayoutImage], input);
     const int tot = b->total[kGatherBins], rtot = b->total[kReduceBins];
     // one-wave workgroups: four times the workgroups of the 256-thread phases
     const int bgrid = std::max(1, std::min(4 * grid_for(b, tot), tot));
+    if (b->fwd_prefix && op != DD_BPROJ_TW) {
+      // running sums of every line, then one lookup per (bin, line); the
+      // weighted transpose keeps the pixel walk (its weight varies per pixel)
+      ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
+      const int ptot = b->total[kLayoutPrefix];
+      hipLaunchKernelGGL(k_batch_prefix, dim3(grid_for(b, ptot)), dim3(kThreads), 0, st, inst,
+                         n, ptot, input);
+      if (op == DD_PROJ) {
+        hipLaunchKernelGGL((k_batch_bins<W_FP, true>), dim3(bgrid), dim3(kBinsWave), 0, st,
+                           inst, n, tot);
+        hipLaunchKernelGGL((k_batch_reduce<W_FP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
+                           st, inst, n, rtot, output);
+      } else {
+        hipLaunchKernelGGL((k_batch_bins<W_BP, true>), dim3(bgrid), dim3(kBinsWave), 0, st,
+                           inst, n, tot);
+        hipLaunchKernelGGL((k_batch_reduce<W_BP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
+                           st, inst, n, rtot, output);
+      }
+      return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+    }
+    hipLaunchKernelGGL((k_batch_layout<false>), dim3(grid_for(b, b->total[kLayoutImage])),
+                       dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutImage),
+                       b->total[kLayoutImage], input);
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ) {
-      hipLaunchKernelGGL((k_batch_bins<W_FP>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
+      hipLaunchKernelGGL((k_batch_bins<W_FP, false>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
                          tot);
       hipLaunchKernelGGL((k_batch_reduce<W_FP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
                          st, inst, n, rtot, output);
     } else if (op == DD_BPROJ_T) {
-      hipLaunchKernelGGL((k_batch_bins<W_BP>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
+      hipLaunchKernelGGL((k_batch_bins<W_BP, false>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
                          tot);
       hipLaunchKernelGGL((k_batch_reduce<W_BP>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
                          st, inst, n, rtot, output);
     } else {
-      hipLaunchKernelGGL((k_batch_bins<W_BPW>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
+      hipLaunchKernelGGL((k_batch_bins<W_BPW, false>), dim3(bgrid), dim3(kBinsWave), 0, st, inst, n,
                          tot);
       hipLaunchKernelGGL((k_batch_reduce<W_BPW>), dim3(grid_for(b, rtot)), dim3(kThreads), 0,
                          st, inst, n, rtot, output);
     }
   } else {
+    if (b->variant == DD_ADJ_BLOCK_PREFIX && op != DD_BPROJ_W) {
+      // running sums of every view, then one merge walk per 4-pixel run
+      ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
+      const int lt = b->total[kLayoutSino], qt = b->total[kGatherQuads];
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL(k_batch_prefix_sino<true>, dim3(grid_for(b, lt)), dim3(kThreads), 0,
+                           st, inst, n, lt, input);
+      else
+        hipLaunchKernelGGL(k_batch_prefix_sino<false>, dim3(grid_for(b, lt)), dim3(kThreads), 0,
+                           st, inst, n, lt, input);
+      if (op == DD_PROJ_T)
+        hipLaunchKernelGGL((k_batch_quads_prefix<W_FP>), dim3(grid_for(b, qt)),
+                           dim3(kQuadThreads), 0, st, inst, n, qt, output);
+      else
+        hipLaunchKernelGGL((k_batch_quads_prefix<W_BP>), dim3(grid_for(b, qt)),
+                           dim3(kQuadThreads), 0, st, inst, n, qt, output);
+      return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
+    }
     if (op == DD_PROJ_T)
       hipLaunchKernelGGL((k_batch_layout<true>), dim3(grid_for(b, b->total[kLayoutSino])),
                          dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutSino),
@@ -499,11 +716,12 @@ dd_status dd2d_batch_run(const dd2d_batch* b, int op, const float* input, float*
                          dim3(kThreads), 0, st, inst, n, static_cast<int>(kLayoutSino),
                          b->total[kLayoutSino], input);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (b->variant == DD_ADJ_QUAD || b->variant == DD_ADJ_BLOCK_WINDOW) {
+    if (b->variant == DD_ADJ_QUAD || b->variant == DD_ADJ_BLOCK_WINDOW ||
+        b->variant == DD_ADJ_BLOCK_PREFIX) {
       const int qt = b->total[kGatherQuads];
       if (b->variant == DD_ADJ_QUAD)
         launch_quads<0>(b, op, inst, n, qt, output, st);
-      else
+      else  // DD_ADJ_BLOCK_WINDOW, and BPROJ_W of DD_ADJ_BLOCK_PREFIX
         launch_quads<kBlockWindow>(b, op, inst, n, qt, output, st);
       return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
     }

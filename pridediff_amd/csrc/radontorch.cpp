@@ -679,6 +679,8 @@ PYBIND11_MODULE(radontorch, m) {
   m.attr("ADJ_TILE8") = static_cast<int>(DD_ADJ_TILE8);
   m.attr("ADJ_QUAD") = static_cast<int>(DD_ADJ_QUAD);
   m.attr("ADJ_BLOCK_WINDOW") = static_cast<int>(DD_ADJ_BLOCK_WINDOW);
+  m.attr("FWD_PREFIX") = static_cast<int>(DD_FWD_PREFIX);
+  m.attr("ADJ_BLOCK_PREFIX") = static_cast<int>(DD_ADJ_BLOCK_PREFIX);
   m.attr("PROJ") = static_cast<int>(DD_PROJ);
   m.attr("PROJ_T") = static_cast<int>(DD_PROJ_T);
   m.attr("BPROJ") = static_cast<int>(DD_BPROJ);

@@ -167,6 +167,53 @@ def test_uneven_64_dot_product_per_instance(c2):
     print(f"C2 dot-product, 64 instances: worst rel {worst:.2e}", flush=True)
 
 
+def test_uneven_64_running_sums_vs_fp64_and_dot(c2):
+    """The running-sum gathers (DD_FWD_PREFIX, DD_ADJ_BLOCK_PREFIX) on all 64
+    instances: forward on 4 views each and adjoint of 4 sparse views each vs
+    the FP64 oracle <= 1e-5 (worst printed), and the dot-product test of the
+    pair <= 1e-6 per instance."""
+    from pridediff_amd import load_radontorch
+    rt = load_radontorch()
+    ub, projs, angs, angs_np, imgs = c2
+    x = ub.pack_images([torch.from_numpy(im).cuda() for im in imgs])
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    y = torch.rand(sum(v * d for _, _, v, d in ub.shapes), device="cuda", generator=gen)
+    ys = []
+    for (h, w, v, d) in ub.shapes:
+        yy = torch.zeros(v, d, device="cuda")
+        sel = torch.from_numpy(_c2_sel(v)).cuda()
+        yy[sel] = torch.rand(len(sel), d, device="cuda", generator=gen)
+        ys.append(yy)
+    try:
+        ub.plan.set_variant(rt.ADJ_BLOCK_PREFIX | rt.FWD_PREFIX)
+        with torch.no_grad():
+            sinos = ub.unpack_sinos(ub.projection(x))
+            aty = ub.unpack_images(ub.projection_t(y))
+            backs = ub.unpack_images(ub.projection_t(ub.pack_sinos(ys)))
+    finally:
+        ub.plan.set_variant(rt.ADJ_QUAD)
+    worst_f = worst_a = 0.0
+    for i, ((h, w, v, d), a, im, s, yy, b) in enumerate(zip(ub.shapes, angs_np, imgs, sinos,
+                                                          ys, backs)):
+        sel = _c2_sel(v)
+        g = O.Geom2D(w, h, d, PIX, PIX, DET / SDD, SID, SDD)
+        want = O.op2d("arc", "proj", im[None], a[sel], g, "f64")
+        r = O.rel_l2(s[sel].cpu().numpy()[None], want)
+        worst_f = max(worst_f, r)
+        assert r <= 1e-5, ("forward", i, (h, w, v, d), r)
+        want = O.op2d("arc", "proj_t", yy[sel].cpu().numpy()[None], a[sel], g, "f64")
+        r = O.rel_l2(b.cpu().numpy()[None], want)
+        worst_a = max(worst_a, r)
+        assert r <= 1e-5, ("adjoint", i, (h, w, v, d), r)
+    worst_dot = 0.0
+    for i, (a, yy, xx, b) in enumerate(zip(sinos, ub.unpack_sinos(y), ub.unpack_images(x), aty)):
+        rel, lhs, rhs = dot_rel([a], [yy], [xx], [b])
+        worst_dot = max(worst_dot, rel)
+        assert rel <= 1e-6, (i, ub.shapes[i], lhs, rhs)
+    print(f"C2 running sums, 64 instances x 4 views vs FP64: forward worst rel_l2 {worst_f:.3e}, "
+          f"adjoint {worst_a:.3e}; dot-product worst rel {worst_dot:.2e}", flush=True)
+
+
 def test_uneven_64_small_grid_strides_items(c2):
     """A grid far smaller than the item count makes every workgroup stride
     over many items (and reuse its LDS across them): results equal the
