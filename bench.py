@@ -180,7 +180,7 @@ def summarize_counters(base, out_path):
         return vals, dur
 
     def find(vals, pat):
-        ks = [k for k in vals if k.startswith(pat)]
+        ks = [k for k in vals if k == pat] or [k for k in vals if k.startswith(pat)]
         if len(ks) > 1:
             raise SystemExit(f"counter summary: {pat!r} matches {ks}")
         return ks[0] if ks else None
