@@ -74,6 +74,14 @@ dd_status dd3d_run(int scan, int op, const dd_geom3d* g, const float* input,
  * No counterpart in the reference, whose adjoint walks rows and slices with
  * atomics (ConeFlatDisCUDA3d.cu:284-536). */
 void dd3d_set_compensated(int mode);
+/* Whether the sinogram -> volume gather stages each view's row running sums
+ * for its 16 x 16 pixel tile in LDS: 0 never, 1 always, 2 (default)
+ * automatic — with compensated sums or batches of 1-2 (C3-like cone-flat:
+ * adjoint 20.0 -> 17.1 ms), not for rounded sums at batches of 3+ (helical
+ * 512x512x64 x 4: 94 ms unstaged, 105 ms staged).  The same bits either
+ * way; a view whose block exceeds 32 KB of LDS is read from global memory.
+ * No counterpart in the reference. */
+void dd3d_set_adjoint_staging(int mode);
 
 #ifdef __cplusplus
 }

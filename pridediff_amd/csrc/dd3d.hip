@@ -52,6 +52,18 @@ bool use_compensated(int bt) {
   return m == 1 || (m == 2 && bt <= 2);
 }
 
+// Whether k_voxels3d stages each view's row running sums for its pixel tile
+// in LDS (dd3d_set_adjoint_staging; the same bits either way): 0 never, 1
+// always, 2 automatic: with compensated sums or instance tiles of 1-2 (16
+// slices per thread), where it measured faster; at tile 4 with rounded sums
+// (8 slices per thread, less reuse per staged view) global reads win.
+std::atomic<int> g_stage3{2};
+
+bool use_stage3(int bt, bool comp) {
+  const int m = g_stage3.load();
+  return m == 1 || (m == 2 && (comp || bt <= 2));
+}
+
 enum WMode3 { W3_FP = 0, W3_BP = 1, W3_BPW = 2 };
 
 struct Geo3 {
@@ -625,20 +637,94 @@ __device__ __forceinline__ int guess_col_fast(const Geo3& g, const ViewGeo3& vg,
   return static_cast<int>(kf);
 }
 
-template <int BT, int WM, bool COMP>
+// Columns [kb, ke) that hold every pixel edge of one 16 x 16 pixel tile in
+// one view, and the range of their magnifications (k_voxels3d's staging).
+struct VoxRange {
+  double mmin, mmax;
+  int kb, ke;
+};
+
+// One thread per (view, pixel tile).  The tile's axis extent is spanned by
+// its corners (a pixel edge's axis point is monotone along and across the
+// lines), widened by one column edge each side against rounding.
+__global__ __launch_bounds__(kThreads3) void k_voxel_ranges(
+    Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
+    const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
+    VoxRange* __restrict__ out) {
+  const int ntx = (g.W + kTile3 - 1) / kTile3;
+  const int tiles = ntx * ((g.H + kTile3 - 1) / kTile3);
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(tiles) * g.V) return;
+  const int v = static_cast<int>(idx / tiles), t = static_cast<int>(idx % tiles);
+  const int c0 = (t % ntx) * kTile3, r0 = (t / ntx) * kTile3;
+  const int c1 = min(c0 + kTile3, g.W), r1 = min(r0 + kTile3, g.H);
+  const ViewGeo3 vg = views[v];
+  const LineGeo3* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
+  double amin = 1e300, amax = -1e300;
+  auto corner = [&](int l, int i) {
+    const double a = lv[l].a0 + i / lv[l].inv_p;
+    amin = fmin(amin, a);
+    amax = fmax(amax, a);
+  };
+  if (vg.xaxis) {  // lines = rows, pixel edges = columns
+    corner(r0, c0), corner(r0, c1), corner(r1 - 1, c0), corner(r1 - 1, c1);
+  } else {  // lines = columns, pixel edges from the bottom
+    corner(c0, g.H - r1), corner(c0, g.H - r0), corner(c1 - 1, g.H - r1),
+        corner(c1 - 1, g.H - r0);
+  }
+  const double* __restrict__ ce = cedge + static_cast<size_t>(v) * (g.C + 1);
+  int lo = 0, hi = g.C;  // last edge <= amin
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (ce[mid] <= amin) lo = mid; else hi = mid - 1;
+  }
+  const int kb = max(lo - 1, 0);
+  lo = 0, hi = g.C;  // first edge >= amax
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (ce[mid] >= amax) hi = mid; else lo = mid + 1;
+  }
+  const int ke = max(min(lo + 1, g.C), kb + 1);
+  double mmin = 1e300, mmax = -1e300;
+  for (int k = kb; k < ke; ++k) {
+    const double m = mag[static_cast<size_t>(v) * g.C + k];
+    mmin = fmin(mmin, m);
+    mmax = fmax(mmax, m);
+  }
+  out[idx] = VoxRange{mmin, mmax, kb, ke};
+}
+
+// Floats of LDS the staged k_voxels3d holds one view's rows x columns of
+// running sums in (32 KB).
+constexpr int kStage3Floats = 8192;
+
+// STG: per view, the block of running sums the workgroup's 16 x 16 pixel
+// tile and slice chunk can read is staged in LDS first (rows x columns from
+// the view's column range and the chunk's z extent; two barriers per view),
+// and every evaluation reads it there; a view whose block exceeds the
+// buffer reads global memory.  The values are copies, so the bits are those
+// of the unstaged gather.
+template <int BT, int WM, bool COMP, bool STG>
 __global__ __launch_bounds__(kThreads3)
 __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
     const float* __restrict__ sino, const float* __restrict__ sino2,
-    float* __restrict__ out) {
+    float* __restrict__ out, const VoxRange* __restrict__ ranges) {
   constexpr int S = slices3<BT>();
+  // staged view: rows [srb, srb + sn / sncols) x columns [skb, skb + sncols)
+  // of hi (or the rounded sums) in lq[0 ..), (lo, row value) pairs after them
+  __shared__ float lq[STG ? kStage3Floats : 1];
+  constexpr int kEntry = COMP ? 3 * BT : BT;  // floats per (row, column)
   const int ntx = (g.W + kTile3 - 1) / kTile3;
+  const int tile = blockIdx.x;
+  const int tiles = ntx * ((g.H + kTile3 - 1) / kTile3);
   const int col = (blockIdx.x % ntx) * kTile3 + (threadIdx.x % kTile3);
   const int row = (blockIdx.x / ntx) * kTile3 + (threadIdx.x / kTile3);
   const int s0 = blockIdx.y * S;
   const int b0 = blockIdx.z * BT;
-  if (col >= g.W || row >= g.H) return;
+  if (!STG && (col >= g.W || row >= g.H)) return;
+  const bool active = col < g.W && row < g.H;
   float xcw = 0.0f, ycw = 0.0f;
   if constexpr (WM == W3_BPW) pixel_centre3(g, 1, row, col, &xcw, &ycw);
   float acc[S][BT];
@@ -652,6 +738,60 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
   for (int v = 0; v < g.V; ++v) {
     const ViewGeo3 vg = views[v];
     if (zc_hi < vg.zlo || zc_lo > vg.zhi) continue;  // the view's cone misses the chunk
+    // staging (block-uniform): the rows the chunk's slices can reach on the
+    // tile's lines through its columns.  A row coordinate r = ((z - sz) / m
+    // + sz - zdet0) / rowp is monotone in the slice edge's z (extreme on the
+    // first and last line at the chunk's first and last slice edge) and in
+    // the column magnification m, so the box corners bound it; one row of
+    // margin each side (two above: rows r and r + 1 are read).
+    bool staged = false;
+    int srb = 0, skb = 0, sncols = 1, sn = 0;
+    if constexpr (STG) {
+      const VoxRange vr = ranges[static_cast<size_t>(v) * tiles + tile];
+      const int l0 = vg.xaxis ? (blockIdx.x / ntx) * kTile3 : (blockIdx.x % ntx) * kTile3;
+      const int l1 = min(l0 + kTile3, vg.xaxis ? g.H : g.W) - 1;
+      const LineGeo3* __restrict__ lvv = lines + static_cast<size_t>(v) * g.Lmax;
+      double zmin = 1e300, zmax = -1e300;
+      for (int e = 0; e < 2; ++e) {
+        const LineGeo3 lg = lvv[e ? l1 : l0];
+        zmin = fmin(zmin, fma(static_cast<double>(s0), lg.pz, lg.z0));
+        zmax = fmax(zmax, fma(static_cast<double>(s0 + S), lg.pz, lg.z0));
+      }
+      double rmin = 1e300, rmax = -1e300;
+      for (int e = 0; e < 4; ++e) {
+        const double z = (e & 1) ? zmax : zmin, m = (e & 2) ? vr.mmax : vr.mmin;
+        const double r = ((z - vg.sz) / m + vg.sz - vg.zdet0) / vg.rowp;
+        rmin = fmin(rmin, r);
+        rmax = fmax(rmax, r);
+      }
+      srb = static_cast<int>(fmin(fmax(floor(rmin) - 1.0, 0.0), static_cast<double>(g.R + 1)));
+      const int sre =
+          static_cast<int>(fmin(fmax(floor(rmax) + 2.0, 0.0), static_cast<double>(g.R + 1)));
+      skb = vr.kb;
+      sncols = vr.ke - vr.kb;
+      sn = (sre - srb + 1) * sncols;
+      staged = sn * kEntry <= kStage3Floats;
+      __syncthreads();  // the previous view's walks are done with lq
+      if (staged) {
+        const size_t vblock = static_cast<size_t>(v) * (g.R + 2) * g.C * g.Bp;
+        for (int e = threadIdx.x; e < sn; e += kThreads3) {
+          const size_t src = vblock + (static_cast<size_t>(srb + e / sncols) * g.C + skb +
+                                       e % sncols) * g.Bp + b0;
+          float h[BT];
+          load_tile<BT>(sino + src, h);
+#pragma unroll
+          for (int b = 0; b < BT; ++b) lq[e * BT + b] = h[b];
+          if constexpr (COMP) {
+            float lx[2 * BT];
+            load_tile<2 * BT>(sino2 + 2 * src, lx);
+#pragma unroll
+            for (int b = 0; b < 2 * BT; ++b) lq[sn * BT + e * 2 * BT + b] = lx[b];
+          }
+        }
+      }
+      __syncthreads();
+      if (!active) continue;
+    }
     int l, i, N;
     if (vg.xaxis) {
       l = row;
@@ -698,7 +838,62 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
       const float du = static_cast<float>(u0 - rb);
       const float invb = static_cast<float>(inv_b);
       const float ulo = static_cast<float>(-rbase), uhi = static_cast<float>(g.R - rbase);
-      if (uhi > 0.0f && fmaf(static_cast<float>(S), invb, du) > ulo) {
+      if (STG && staged && uhi > 0.0f && fmaf(static_cast<float>(S), invb, du) > ulo) {
+        // the same evaluations as below, reading the staged block: element
+        // (row rbase + fu, column k) of the hi block at lb + fu * rsl
+        const float wxb = wx * static_cast<float>(z.B);
+        const int lb = ((rbase - srb) * sncols + (k - skb)) * BT;
+        const int rsl = sncols * BT;
+        const __attribute__((address_space(3))) float* lh =
+            (const __attribute__((address_space(3))) float*)(lq);
+        if constexpr (COMP) {
+          auto eval = [&](int j, float (&h)[BT], float (&m)[BT]) {
+            const float u = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(j), invb, du), ulo, uhi);
+            const float fu = floorf(u);
+            const float fr = u - fu;
+            const int e = lb + static_cast<int>(fu) * rsl;
+#pragma unroll
+            for (int b = 0; b < BT; ++b) {
+              h[b] = lh[e + b];
+              m[b] = fmaf(fr, lh[sn * BT + 2 * (e + b) + 1], lh[sn * BT + 2 * (e + b)]);
+            }
+          };
+          float hprev[BT], mprev[BT];
+          eval(0, hprev, mprev);
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            float hnext[BT], mnext[BT];
+            eval(j + 1, hnext, mnext);
+#pragma unroll
+            for (int b = 0; b < BT; ++b) {
+              acc[j][b] = fmaf(wxb, (hnext[b] - hprev[b]) + (mnext[b] - mprev[b]), acc[j][b]);
+              hprev[b] = hnext[b];
+              mprev[b] = mnext[b];
+            }
+          }
+        } else {
+          auto eval = [&](int j, float (&l)[BT]) {
+            const float u = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(j), invb, du), ulo, uhi);
+            const float fu = floorf(u);
+            const float fr = u - fu;
+            const int e = lb + static_cast<int>(fu) * rsl;
+#pragma unroll
+            for (int b = 0; b < BT; ++b) l[b] = fmaf(fr, lh[e + rsl + b] - lh[e + b], lh[e + b]);
+          };
+          float lprev[BT];
+          eval(0, lprev);
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            float lnext[BT];
+            eval(j + 1, lnext);
+#pragma unroll
+            for (int b = 0; b < BT; ++b) {
+              acc[j][b] = fmaf(wxb, lnext[b] - lprev[b], acc[j][b]);
+              lprev[b] = lnext[b];
+            }
+          }
+        }
+      } else if (uhi > 0.0f && fmaf(static_cast<float>(S), invb, du) > ulo) {
         const float wxb = wx * static_cast<float>(z.B);
         // the view's block of Q (wave-uniform base) plus 32-bit byte
         // offsets: a block holds (R + 2) C Bp < 2^30 floats (checked on the
@@ -773,6 +968,7 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
       p2 = edge_pos(ce[k + 1], lg.a0, lg.inv_p, N);
     }
   }
+  if (!active) return;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     const int sl = s0 + j;
@@ -792,7 +988,7 @@ constexpr size_t kAlign3 = 256;
 
 struct Layout3 {
   size_t views, cedge, mag, lines, geo_total;
-  size_t buf0, buf1, data_total;
+  size_t buf0, buf1, ranges, data_total;
 };
 
 bool to_sino3(int op) {
@@ -829,6 +1025,9 @@ Layout3 make_layout3(const Geo3& g, int op) {
     L.buf0 = take(q);
     // compensated running sums: a second plane of (lo, row value) pairs
     L.buf1 = use_compensated(g.BT) ? take(2 * q) : L.buf0;
+    // column ranges of the adjoint's LDS staging, per (view, 16 x 16 tile)
+    L.ranges = take(sizeof(VoxRange) * g.V * ((g.W + kTile3 - 1) / kTile3) *
+                    ((g.H + kTile3 - 1) / kTile3));
   }
   L.data_total = off;
   return L;
@@ -871,10 +1070,26 @@ bool fill_geo3(Geo3* out, int scan, const dd_geom3d* g, int batch, int views) {
 }
 
 // sinogram -> volume: running sums over rows, then the voxel gather
+template <int BT, bool COMP, bool STG>
+void launch_voxels3(const Geo3& g, int op, dim3 grid, const ViewGeo3* views,
+                    const double* cedge, const double* mag, const LineGeo3* lines, float* q,
+                    float* q2, float* output, VoxRange* ranges, hipStream_t st) {
+  if (op == DD_PROJ_T)
+    hipLaunchKernelGGL((k_voxels3d<BT, W3_FP, COMP, STG>), grid, dim3(kThreads3), 0, st, g,
+                       views, cedge, mag, lines, q, q2, output, ranges);
+  else if (op == DD_BPROJ)
+    hipLaunchKernelGGL((k_voxels3d<BT, W3_BP, COMP, STG>), grid, dim3(kThreads3), 0, st, g,
+                       views, cedge, mag, lines, q, q2, output, ranges);
+  else
+    hipLaunchKernelGGL((k_voxels3d<BT, W3_BPW, COMP, STG>), grid, dim3(kThreads3), 0, st, g,
+                       views, cedge, mag, lines, q, q2, output, ranges);
+}
+
 template <int BT, bool COMP>
 void launch_adjoint3(const Geo3& g, int op, const float* input, float* output,
                      const ViewGeo3* views, const double* cedge, const double* mag,
-                     const LineGeo3* lines, float* q, float* q2, hipStream_t st) {
+                     const LineGeo3* lines, float* q, float* q2, VoxRange* ranges,
+                     hipStream_t st) {
   dim3 sgrid((g.C + kThreads3 - 1) / kThreads3 * g.V, 1, g.Bp / BT);
   if (op == DD_PROJ_T)
     hipLaunchKernelGGL((k_prefix_sino3d<BT, true, COMP>), sgrid, dim3(kThreads3), 0, st, g,
@@ -886,15 +1101,16 @@ void launch_adjoint3(const Geo3& g, int op, const float* input, float* output,
   constexpr int S = slices3<BT>();
   dim3 grid(ntx * nty, (g.Dz + S - 1) / S, g.Bp / BT);
   ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-  if (op == DD_PROJ_T)
-    hipLaunchKernelGGL((k_voxels3d<BT, W3_FP, COMP>), grid, dim3(kThreads3), 0, st, g, views,
-                       cedge, mag, lines, q, q2, output);
-  else if (op == DD_BPROJ)
-    hipLaunchKernelGGL((k_voxels3d<BT, W3_BP, COMP>), grid, dim3(kThreads3), 0, st, g, views,
-                       cedge, mag, lines, q, q2, output);
-  else
-    hipLaunchKernelGGL((k_voxels3d<BT, W3_BPW, COMP>), grid, dim3(kThreads3), 0, st, g, views,
-                       cedge, mag, lines, q, q2, output);
+  if (use_stage3(BT, COMP)) {
+    const long long n = 1LL * ntx * nty * g.V;
+    hipLaunchKernelGGL(k_voxel_ranges, dim3(static_cast<unsigned>((n + kThreads3 - 1) / kThreads3)),
+                       dim3(kThreads3), 0, st, g, views, cedge, mag, lines, ranges);
+    launch_voxels3<BT, COMP, true>(g, op, grid, views, cedge, mag, lines, q, q2, output, ranges,
+                                   st);
+  } else {
+    launch_voxels3<BT, COMP, false>(g, op, grid, views, cedge, mag, lines, q, q2, output,
+                                    ranges, st);
+  }
 }
 
 template <int BT>
@@ -925,10 +1141,14 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
     else
       hipLaunchKernelGGL((k_bins3d<BT, W3_BPW>), grid, dim3(kThreads3), 0, st, g,
                          views, cedge, mag, lines, buf0, buf1, output);
-  } else if (buf1 != buf0) {
-    launch_adjoint3<BT, true>(g, op, input, output, views, cedge, mag, lines, buf0, buf1, st);
   } else {
-    launch_adjoint3<BT, false>(g, op, input, output, views, cedge, mag, lines, buf0, buf1, st);
+    auto* ranges = reinterpret_cast<VoxRange*>(ws + L.ranges);
+    if (buf1 != buf0)
+      launch_adjoint3<BT, true>(g, op, input, output, views, cedge, mag, lines, buf0, buf1,
+                                ranges, st);
+    else
+      launch_adjoint3<BT, false>(g, op, input, output, views, cedge, mag, lines, buf0, buf1,
+                                 ranges, st);
   }
   return hipGetLastError() == hipSuccess ? DD_OK : DD_LAUNCH_FAIL;
 }
@@ -943,6 +1163,8 @@ extern "C" {
 void dd3d_set_compensated(int mode) {
   g_compensated = (mode == 0 || mode == 1) ? mode : 2;
 }
+
+void dd3d_set_adjoint_staging(int mode) { g_stage3 = (mode == 0 || mode == 1) ? mode : 2; }
 
 size_t dd3d_workspace_size(int scan, int op, const dd_geom3d* g, int batch,
                            int views) {

@@ -674,6 +674,8 @@ PYBIND11_MODULE(radontorch, m) {
   m.def("set_adjoint_variant", [](int v) { dd2d_set_adjoint_variant(v); });
   m.def("set_compensated3d", [](int mode) { dd3d_set_compensated(mode); },
         pybind11::arg("mode"));
+  m.def("set_adjoint_staging3d", [](int on) { dd3d_set_adjoint_staging(on); },
+        pybind11::arg("on"));
   m.attr("ADJ_PER_PIXEL") = static_cast<int>(DD_ADJ_PER_PIXEL);
   m.attr("ADJ_TILE16") = static_cast<int>(DD_ADJ_TILE16);
   m.attr("ADJ_TILE8") = static_cast<int>(DD_ADJ_TILE8);
