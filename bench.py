@@ -147,9 +147,9 @@ COUNTER_KEYS = {
                "uneven64:k_batch_bins": ["k_batch_bins<0, false>", "k_batch_reduce<0>"],
                "uneven64:k_batch_bins_prefix": ["k_batch_prefix", "k_batch_bins<0, true>",
                                                 "k_batch_reduce<0>"]},
-    "cone_flat": {"cone_flat:k_voxels3d": ["k_voxels3d<2, 0, true>"],
+    "cone_flat": {"cone_flat:k_voxels3d": ["k_voxel_ranges", "k_voxels3d<2, 0, true, true>"],
                   "cone_flat:k_bins3d": ["k_bins3d<2, 0>"]},
-    "helical": {"helical:k_voxels3d": ["k_voxels3d<4, 0, false>"],
+    "helical": {"helical:k_voxels3d": ["k_voxels3d<4, 0, false, false>"],
                 "helical:k_bins3d": ["k_bins3d<4, 0>"]},
 }
 
@@ -512,6 +512,8 @@ def bench3d(torch, kind, steps, warmup, check, dc_loop=True):
         rt.set_compensated3d(2)
         acc["default_mode"] = ("compensated" if Bn <= 2 else "rounded") + \
             " (automatic: compensated at instance tiles 1-2)"
+        acc["lds_staging"] = ("on" if Bn <= 2 else "on for compensated sums only") + \
+            " (automatic: compensated sums or instance tiles 1-2; same bits either way)"
         out["accuracy"] = acc
     return out
 
