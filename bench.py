@@ -606,17 +606,19 @@ def cpu_baseline(ang_np, x_np, y_np):
     g = O.Geom2D(N, N, D, PIX, PIX, DET / SDD, SID, SDD)
     mode = "ref" if O.ref_available() else "f32"
     xs, ys = x_np[:2], y_np[:2]
-    t = _cpu_time(lambda: (O.op2d("arc", "proj", xs, ang_np, g, mode),
-                           O.op2d("arc", "proj_t", ys, ang_np, g, mode)))
+    t_fp = _cpu_time(lambda: O.op2d("arc", "proj", xs, ang_np, g, mode))
+    t_pt = _cpu_time(lambda: O.op2d("arc", "proj_t", ys, ang_np, g, mode))
+    t = t_fp + t_pt
     threads, hw = _ref_threads(V, mode)
     return {
         "value": 2 * V * D / t, "unit": "detector samples/s", "cores": threads,
         "hardware_concurrency": hw,
         "kind": "reference" if mode == "ref" else "port",
+        "per_slice_s": {"proj": t_fp / 2, "proj_t": t_pt / 2, "proj_plus_proj_t": t / 2},
         "sample": (f"2 slices of the 512^2 fan-arc workload (720 views x 736 bins), "
-                   f"Proj + ProjTrans, median of 3 runs ({t:.2f} s each); deeprecon's CPU "
-                   f"thread policy min(ceil(V/128), cores) = {threads} threads on a "
-                   f"{hw}-CPU host"),
+                   f"Proj and ProjTrans each the median of 3 runs ({t_fp:.2f} s + {t_pt:.2f} s); "
+                   f"deeprecon's CPU thread policy min(ceil(V/128), cores) = {threads} threads "
+                   f"on a {hw}-CPU host"),
     }
 
 
