@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-3d", action="store_true",
                     help="skip the cone-flat and helical legs of the default run")
     ap.add_argument("--no-c0", action="store_true", help="skip the parallel-beam CPU-case leg")
+    ap.add_argument("--no-fidelity", action="store_true",
+                    help="skip the data-consistency step leg of the default run")
     ap.add_argument("--no-dc-loop", action="store_true",
                     help="skip the helical workload's 50-step descent loop (profiling runs)")
     ap.add_argument("--no-tune", action="store_true",
@@ -251,6 +253,20 @@ def timed(torch, fn, steps, warmup):
         fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def timed_in_turn(torch, fns, steps, warmup, rounds=3):
+    """Median over ``rounds`` of each function's time per call, the functions
+    timed one after another in every round, so that a clock still ramping up
+    during the first measurements does not favour the ones timed last."""
+    for fn in fns:
+        for _ in range(warmup):
+            fn()
+    res = [[] for _ in fns]
+    for _ in range(rounds):
+        for i, fn in enumerate(fns):
+            res[i].append(timed(torch, fn, steps, 0))
+    return [statistics.median(r) for r in res]
 
 
 def uneven_c2(torch, dev, steps, warmup, check, tune_grid=True, cpu=False, variant=None):
@@ -518,34 +534,50 @@ def bench3d(torch, kind, steps, warmup, check, dc_loop=True):
     return out
 
 
-def bench_fidelity(torch, steps, warmup, check):
+def bench_fidelity(torch, steps, warmup, ctx=None):
     """PrideDiff's data-consistency step x - w * FBP(A x - y)
     (model/pridediff.py:56-68) on the 512^2 fan-arc batch of 16: the fused
     planned step (residual and update in the projector kernels' writes) in
     one HIP graph, vs the unfused planned step in a graph, vs the drop-in
     module called on the batch (GEMM filter), vs the same with the
-    reference's conv2d filter."""
+    reference's conv2d filter.  ``ctx`` = (projector, angles, x0, y, tuned
+    batched projector) of the headline run: then only the fused, unfused and
+    module steps are timed (the compact leg of the default run)."""
     from pridediff_amd.autotune import tune_uniform
     from pridediff_amd.batched import GraphStep
     from pridediff_amd.fbp import PlannedFBP2d, fidelity_step, fidelity_step_fused
 
     dev = torch.device("cuda:0")
-    P, ang, ang_np, x_true, x0, y = workload(torch, dev)
-    BP = tune_uniform(P, ang, B)
+    compact = ctx is not None
+    if compact:
+        P, ang, x0, y, BP = ctx
+    else:
+        P, ang, ang_np, x_true, x0, y = workload(torch, dev)
+        BP = tune_uniform(P, ang, B)
     fbp = PlannedFBP2d(P, ang, B, plan=BP)
     w = torch.tensor(0.5, device=dev)
     with torch.no_grad():
         # fused: residual in the forward's reduction, update in the BP's write
         fused = GraphStep(lambda a, c: fidelity_step_fused(fbp, a, c, w.reshape(1)), x0, y)
-        t_graph = timed(torch, lambda: fused.replay(x0, y), steps, warmup)
-        out_graph = fused.replay(x0, y).clone()
         step = GraphStep(lambda a, c: fidelity_step(BP.projection, fbp, a, c, w), x0, y)
-        t_unfused = timed(torch, lambda: step.replay(x0, y), steps, warmup)
-        out_unfused = step.replay(x0, y).clone()
-        t_module = timed(torch, lambda: x0 - w * P.filtered_backprojection(
-            P.projection(x0, ang) - y, ang), max(3, steps // 4), 1)
-        out_module = x0 - w * P.filtered_backprojection(P.projection(x0, ang) - y, ang)
 
+        def module():
+            return x0 - w * P.filtered_backprojection(P.projection(x0, ang) - y, ang)
+        t_graph, t_unfused, t_module = timed_in_turn(
+            torch, (lambda: fused.replay(x0, y), lambda: step.replay(x0, y), module),
+            steps, warmup)
+        out_graph = fused.replay(x0, y).clone()
+        out_unfused = step.replay(x0, y).clone()
+        out_module = module()
+    rel = lambda a, b: float((a - b).double().norm() / b.double().norm())
+    if compact:
+        return {"workload": "x - w FBP(A x - y) on the same batch",
+                "fused_graph_ms": t_graph * 1e3, "unfused_graph_ms": t_unfused * 1e3,
+                "module_call_ms": t_module * 1e3,
+                "value": B * V * D / t_graph, "unit": "detector samples/s",
+                "agreement": {"fused_vs_unfused_rel_l2": rel(out_graph, out_unfused),
+                              "graph_vs_module_rel_l2": rel(out_graph, out_module)}}
+    with torch.no_grad():
         def conv_fbp(sino):
             f = torch.nn.functional.conv2d(sino * P.weight, P.filter, padding=(0, D - 1))
             r = P.backprojection_w(f, ang) * (ang[-1] - ang[0]).abs() / (V - 1)
@@ -570,13 +602,9 @@ def bench_fidelity(torch, steps, warmup, check):
         "module_call_ms": t_module * 1e3,
         "module_call_conv2d_filter_ms": t_conv * 1e3,
         "filter_stage_ms": {"gemm": t_gemm_only * 1e3, "conv2d": t_conv_only * 1e3},
-        "agreement": {
-            "fused_vs_unfused_rel_l2": float((out_graph - out_unfused).double().norm()
-                                             / out_unfused.double().norm()),
-            "graph_vs_module_rel_l2": float((out_graph - out_module).double().norm()
-                                            / out_module.double().norm()),
-            "graph_vs_conv2d_rel_l2": float((out_graph - out_conv).double().norm()
-                                            / out_conv.double().norm())},
+        "agreement": {"fused_vs_unfused_rel_l2": rel(out_graph, out_unfused),
+                      "graph_vs_module_rel_l2": rel(out_graph, out_module),
+                      "graph_vs_conv2d_rel_l2": rel(out_graph, out_conv)},
         "cpu_baseline": None,
     }
     return out
@@ -734,8 +762,7 @@ def main():
             "uneven_c2": c2, "cpu_baseline": c2.pop("cpu_sample", None)}), flush=True)
         return
     if args.workload == "fidelity":
-        print(json.dumps(bench_fidelity(torch, args.steps, args.warmup, not args.no_accuracy)),
-              flush=True)
+        print(json.dumps(bench_fidelity(torch, args.steps, args.warmup)), flush=True)
         return
     if args.workload != "fanarc":
         print(json.dumps(bench3d(torch, args.workload, args.steps, args.warmup,
@@ -836,6 +863,8 @@ def main():
             legs[kind] = {k: r[k] for k in ("config", "ms_per_step", "value", "roofline", "kernels",
                                             "accuracy", "dc_loop") if k in r}
     c0 = None if args.no_c0 else parallel_c0(torch, dev, not args.no_accuracy)
+    fid = None if args.no_fidelity else bench_fidelity(torch, max(5, args.steps // 2), 2,
+                                                       ctx=(P, ang, x0, y, BP))
     cpu = None if args.no_cpu else cpu_baseline(ang_np, x_np, y_np)
     samples = B * V * D
     out = {
@@ -864,6 +893,7 @@ def main():
         "cone_flat_c3": legs.get("cone_flat"),
         "helical_c4": legs.get("helical"),
         "parallel_c0": c0,
+        "fidelity_step": fid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
