@@ -466,8 +466,12 @@ struct RowEdges {
 };
 
 // Rows (forward) and slices (adjoint) one thread carries in registers.
+// Forward: 8 rows per thread at instance tiles 1-2, 4 at tile 4 (92 VGPRs,
+// 5 waves per SIMD); against 4 / 2 the per-line geometry work and the slice
+// a chunk boundary splits are shared by twice the rows, and 16 / 8 (120-124
+// VGPRs, 4 waves) loses more in occupancy than it saves.
 #ifndef DD3D_FWD_ROWS
-#define DD3D_FWD_ROWS 4
+#define DD3D_FWD_ROWS 8
 #endif
 #ifndef DD3D_ADJ_SLICES
 #define DD3D_ADJ_SLICES 16
