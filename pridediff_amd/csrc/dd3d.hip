@@ -36,8 +36,15 @@ namespace dd {
 namespace {
 
 constexpr int kThreads3 = 256;
-constexpr int kColsPerBlock = 64;  // k_bins3d: 64 columns x 4 row chunks
-constexpr int kRowsPerBlock = 4;
+// k_bins3d workgroups: 64 detector columns x DD3D_FWD_CHUNKS row chunks (one
+// wave per chunk).  Two chunks measured best: C4 forward 77.6 ms at four,
+// 75.4 at two, 77.4 at one; C3 17.7 / 17.5 / 17.3 ms.
+#ifndef DD3D_FWD_CHUNKS
+#define DD3D_FWD_CHUNKS 2
+#endif
+constexpr int kColsPerBlock = 64;
+constexpr int kRowsPerBlock = DD3D_FWD_CHUNKS;
+constexpr int kFwdThreads = kColsPerBlock * kRowsPerBlock;
 constexpr int kTile3 = 16;         // k_voxels3d: 16 x 16 pixels x one slice chunk
 
 // Running sums of the sinogram -> volume ops (dd3d_set_compensated):
@@ -499,7 +506,7 @@ constexpr int adj_waves3() {
 // overlaps (slice-major: a row-by-row walk reloads each slice for every
 // row that overlaps it).
 template <int BT, int WM>
-__global__ __launch_bounds__(kThreads3) void k_bins3d(
+__global__ __launch_bounds__(kFwdThreads) void k_bins3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
     const float* __restrict__ xrow, const float* __restrict__ xcol,
@@ -1137,13 +1144,13 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
     dim3 grid(g.V, (g.C + kColsPerBlock - 1) / kColsPerBlock, ncb * ntile);
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
     if (op == DD_PROJ)
-      hipLaunchKernelGGL((k_bins3d<BT, W3_FP>), grid, dim3(kThreads3), 0, st, g,
+      hipLaunchKernelGGL((k_bins3d<BT, W3_FP>), grid, dim3(kFwdThreads), 0, st, g,
                          views, cedge, mag, lines, buf0, buf1, output);
     else if (op == DD_BPROJ_T)
-      hipLaunchKernelGGL((k_bins3d<BT, W3_BP>), grid, dim3(kThreads3), 0, st, g,
+      hipLaunchKernelGGL((k_bins3d<BT, W3_BP>), grid, dim3(kFwdThreads), 0, st, g,
                          views, cedge, mag, lines, buf0, buf1, output);
     else
-      hipLaunchKernelGGL((k_bins3d<BT, W3_BPW>), grid, dim3(kThreads3), 0, st, g,
+      hipLaunchKernelGGL((k_bins3d<BT, W3_BPW>), grid, dim3(kFwdThreads), 0, st, g,
                          views, cedge, mag, lines, buf0, buf1, output);
   } else {
     auto* ranges = reinterpret_cast<VoxRange*>(ws + L.ranges);
