@@ -129,6 +129,10 @@ def roofline_for(key, kernel, ms, b_alg, step_ms=None, n_ops=2):
     if k.get("hbm_bytes"):
         out["hbm"] = {"achieved": k["hbm_bytes"] / t / 1e9, "peak": HBM_PEAK / 1e9,
                       "frac": k["hbm_bytes"] / t / HBM_PEAK}
+    if k.get("ta_busy_frac") is not None:
+        # busy fractions of the two halves of the vector-memory path in the
+        # counter runs: the addresser (TA) and the data return (TD)
+        out["busy_in_counter_run"] = {"ta": k["ta_busy_frac"], "td": k.get("td_busy_frac")}
     if k:
         out["counts_from"] = ("rocprofv3 --pmc runs of this workload, "
                               "profiles/kernel_counters.md; times measured in this run")
