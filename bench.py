@@ -154,9 +154,9 @@ COUNTER_KEYS = {
                "uneven64:k_batch_bins_prefix": ["k_batch_prefix", "k_batch_bins<0, true>",
                                                 "k_batch_reduce<0>"]},
     "cone_flat": {"cone_flat:k_voxels3d": ["k_voxel_ranges", "k_voxels3d<2, 0, true, true>"],
-                  "cone_flat:k_bins3d": ["k_bins3d<2, 0>"]},
+                  "cone_flat:k_bins3d": ["k_bins3d<2, 0, false>"]},
     "helical": {"helical:k_voxels3d": ["k_voxels3d<4, 0, false, false>"],
-                "helical:k_bins3d": ["k_bins3d<4, 0>"]},
+                "helical:k_bins3d": ["k_bins3d<4, 0, true>"]},
 }
 
 

@@ -82,6 +82,14 @@ void dd3d_set_compensated(int mode);
  * way; a view whose block exceeds 32 KB of LDS is read from global memory.
  * No counterpart in the reference. */
 void dd3d_set_adjoint_staging(int mode);
+/* Dispatch order of the volume -> sinogram gather: 0 views first (two
+ * detector-row chunks per workgroup), 1 detector columns first (one chunk),
+ * 2 (default) automatic: columns first when one instance tile's volume copy
+ * exceeds 192 MB (C4-like helical 512x512x64 x 4, 268 MB: forward 75.4 ->
+ * 65.7 ms), views first otherwise (C3-like cone-flat 256^3 x 2, 134 MB: 17.7
+ * ms views first, 19.5 ms columns first).  The same bits either way.  No
+ * counterpart in the reference. */
+void dd3d_set_forward_order(int mode);
 
 #ifdef __cplusplus
 }

@@ -36,15 +36,27 @@ namespace dd {
 namespace {
 
 constexpr int kThreads3 = 256;
-// k_bins3d workgroups: 64 detector columns x DD3D_FWD_CHUNKS row chunks (one
-// wave per chunk).  Two chunks measured best: C4 forward 77.6 ms at four,
-// 75.4 at two, 77.4 at one; C3 17.7 / 17.5 / 17.3 ms.
-#ifndef DD3D_FWD_CHUNKS
-#define DD3D_FWD_CHUNKS 2
-#endif
+// k_bins3d workgroups: 64 detector columns x one or two row chunks (one wave
+// per chunk), in one of two dispatch orders (dd3d_set_forward_order):
+//  * views first: grid (views, column blocks, row-chunk pairs x instance
+//    tiles), two chunks per workgroup.  Consecutive workgroups are
+//    consecutive views of one block, so the blocks in flight sweep the whole
+//    volume copy; best while that copy stays in the 256 MB Infinity Cache
+//    (C3: 17.7 ms; 17.5 / 17.3 ms at two / one chunk against 4).
+//  * columns first: grid (column blocks, views, row chunks x instance tiles),
+//    one chunk per workgroup.  The workgroups in flight are the column blocks
+//    of a few hundred consecutive views for one row chunk: a band of slices,
+//    and on each XCD a few column wedges of it.  Best once the copy outgrows
+//    the cache (C4, 268 MB: 75.4 -> 65.7 ms; C3 19.5 ms).
 constexpr int kColsPerBlock = 64;
-constexpr int kRowsPerBlock = DD3D_FWD_CHUNKS;
-constexpr int kFwdThreads = kColsPerBlock * kRowsPerBlock;
+constexpr int kFwdThreads = 2 * kColsPerBlock;
+constexpr int kChunksViewsFirst = 2;
+constexpr int kChunksColsFirst = 1;
+
+// Dispatch order of k_bins3d: 0 views first, 1 columns first, 2 automatic
+// (columns first when one instance tile's volume copy exceeds 192 MB, three
+// quarters of the Infinity Cache, and the views fit a grid dimension).
+std::atomic<int> g_fwd_order{2};
 constexpr int kTile3 = 16;         // k_voxels3d: 16 x 16 pixels x one slice chunk
 
 // Running sums of the sinogram -> volume ops (dd3d_set_compensated):
@@ -505,18 +517,20 @@ constexpr int adj_waves3() {
 // every slice's voxels loaded once and spread over the S rows by their z
 // overlaps (slice-major: a row-by-row walk reloads each slice for every
 // row that overlaps it).
-template <int BT, int WM>
+template <int BT, int WM, bool COLS_FIRST>
 __global__ __launch_bounds__(kFwdThreads) void k_bins3d(
     Geo3 g, const ViewGeo3* __restrict__ views, const double* __restrict__ cedge,
     const double* __restrict__ mag, const LineGeo3* __restrict__ lines,
     const float* __restrict__ xrow, const float* __restrict__ xcol,
     float* __restrict__ out) {
   constexpr int S = rows3<BT>();
-  const int v = blockIdx.x;
-  const int k = blockIdx.y * kColsPerBlock + (threadIdx.x % kColsPerBlock);
+  constexpr int CH = COLS_FIRST ? kChunksColsFirst : kChunksViewsFirst;
+  const int v = COLS_FIRST ? blockIdx.y : blockIdx.x;
+  const int k = (COLS_FIRST ? blockIdx.x : blockIdx.y) * kColsPerBlock +
+                (threadIdx.x % kColsPerBlock);
   const int nch = (g.R + S - 1) / S;
-  const int ncb = (nch + kRowsPerBlock - 1) / kRowsPerBlock;
-  const int ch = (blockIdx.z % ncb) * kRowsPerBlock + threadIdx.x / kColsPerBlock;
+  const int ncb = (nch + CH - 1) / CH;
+  const int ch = (blockIdx.z % ncb) * CH + threadIdx.x / kColsPerBlock;
   const int b0 = (blockIdx.z / ncb) * BT;
   const int r0 = ch * S;
   if (k >= g.C || r0 >= g.R) return;
@@ -1124,6 +1138,30 @@ void launch_adjoint3(const Geo3& g, int op, const float* input, float* output,
   }
 }
 
+bool use_cols_first(const Geo3& g, int bt) {
+  const int m = g_fwd_order.load();
+  if (g.V > 65535) return false;  // the views are grid dimension y
+  if (m != 2) return m == 1;
+  const double tile_bytes = 4.0 * g.W * g.H * g.Dz * bt;
+  return tile_bytes > 192.0 * (1 << 20);
+}
+
+template <int BT, bool COLS_FIRST>
+void launch_bins3(const Geo3& g, int op, dim3 grid, int chunks, const ViewGeo3* views,
+                  const double* cedge, const double* mag, const LineGeo3* lines,
+                  const float* buf0, const float* buf1, float* output, hipStream_t st) {
+  const dim3 block(kColsPerBlock * chunks);
+  if (op == DD_PROJ)
+    hipLaunchKernelGGL((k_bins3d<BT, W3_FP, COLS_FIRST>), grid, block, 0, st, g, views, cedge,
+                       mag, lines, buf0, buf1, output);
+  else if (op == DD_BPROJ_T)
+    hipLaunchKernelGGL((k_bins3d<BT, W3_BP, COLS_FIRST>), grid, block, 0, st, g, views, cedge,
+                       mag, lines, buf0, buf1, output);
+  else
+    hipLaunchKernelGGL((k_bins3d<BT, W3_BPW, COLS_FIRST>), grid, block, 0, st, g, views, cedge,
+                       mag, lines, buf0, buf1, output);
+}
+
 template <int BT>
 dd_status launch3(const Geo3& g, int op, const float* input, float* output,
                   char* geo, char* ws, const Layout3& L, hipStream_t st) {
@@ -1140,18 +1178,17 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
                        dim3(kThreads3), 0, st, g, input, buf0, buf1);
     constexpr int S = rows3<BT>();
     const int nch = (g.R + S - 1) / S;
-    const int ncb = (nch + kRowsPerBlock - 1) / kRowsPerBlock;
-    dim3 grid(g.V, (g.C + kColsPerBlock - 1) / kColsPerBlock, ncb * ntile);
+    const unsigned ncolb = (g.C + kColsPerBlock - 1) / kColsPerBlock;
     ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
-    if (op == DD_PROJ)
-      hipLaunchKernelGGL((k_bins3d<BT, W3_FP>), grid, dim3(kFwdThreads), 0, st, g,
-                         views, cedge, mag, lines, buf0, buf1, output);
-    else if (op == DD_BPROJ_T)
-      hipLaunchKernelGGL((k_bins3d<BT, W3_BP>), grid, dim3(kFwdThreads), 0, st, g,
-                         views, cedge, mag, lines, buf0, buf1, output);
-    else
-      hipLaunchKernelGGL((k_bins3d<BT, W3_BPW>), grid, dim3(kFwdThreads), 0, st, g,
-                         views, cedge, mag, lines, buf0, buf1, output);
+    if (use_cols_first(g, BT)) {
+      const int ncb = (nch + kChunksColsFirst - 1) / kChunksColsFirst;
+      launch_bins3<BT, true>(g, op, dim3(ncolb, g.V, ncb * ntile), kChunksColsFirst, views,
+                             cedge, mag, lines, buf0, buf1, output, st);
+    } else {
+      const int ncb = (nch + kChunksViewsFirst - 1) / kChunksViewsFirst;
+      launch_bins3<BT, false>(g, op, dim3(g.V, ncolb, ncb * ntile), kChunksViewsFirst, views,
+                              cedge, mag, lines, buf0, buf1, output, st);
+    }
   } else {
     auto* ranges = reinterpret_cast<VoxRange*>(ws + L.ranges);
     if (buf1 != buf0)
@@ -1176,6 +1213,8 @@ void dd3d_set_compensated(int mode) {
 }
 
 void dd3d_set_adjoint_staging(int mode) { g_stage3 = (mode == 0 || mode == 1) ? mode : 2; }
+
+void dd3d_set_forward_order(int mode) { g_fwd_order = (mode == 0 || mode == 1) ? mode : 2; }
 
 size_t dd3d_workspace_size(int scan, int op, const dd_geom3d* g, int batch,
                            int views) {

@@ -676,6 +676,8 @@ PYBIND11_MODULE(radontorch, m) {
         pybind11::arg("mode"));
   m.def("set_adjoint_staging3d", [](int on) { dd3d_set_adjoint_staging(on); },
         pybind11::arg("on"));
+  m.def("set_forward_order3d", [](int mode) { dd3d_set_forward_order(mode); },
+        pybind11::arg("mode"));
   m.attr("ADJ_PER_PIXEL") = static_cast<int>(DD_ADJ_PER_PIXEL);
   m.attr("ADJ_TILE16") = static_cast<int>(DD_ADJ_TILE16);
   m.attr("ADJ_TILE8") = static_cast<int>(DD_ADJ_TILE8);
