@@ -803,6 +803,14 @@ def main():
 
     grad_plain = torch.cat(plain_step())
     t_plain = timed(torch, plain_step, args.plain_steps, 1)
+    # the same calls with the single-call running-sum adjoint (opt-in variant:
+    # the default keeps the walks, bit-identical to the batched path)
+    rt.set_adjoint_variant(rt.ADJ_BLOCK_PREFIX)
+    try:
+        grad_plain_rs = torch.cat(plain_step())
+        t_plain_rs = timed(torch, plain_step, args.plain_steps, 1)
+    finally:
+        rt.set_adjoint_variant(-1)
 
     # ---- per-kernel timing (HIP events on the launch stream) -------------
     rt.profile_enable(True)
@@ -888,7 +896,13 @@ def main():
                    "image": [N, N], "views": V, "bins": D, "batch": B,
                    "pixel_mm": PIX, "det_mm": DET, "sid_mm": SID, "sdd_mm": SDD},
         "plain": {"ms_per_step": t_plain * 1e3, "value": samples / t_plain,
-                  "batched_speedup_per_slice": t_plain / t_batched},
+                  "batched_speedup_per_slice": t_plain / t_batched,
+                  "running_sum_adjoint": {
+                      "ms_per_step": t_plain_rs * 1e3,
+                      "batched_speedup_per_slice": t_plain_rs / t_batched,
+                      "grad_rel_l2_vs_batched": float(
+                          (grad_batched - grad_plain_rs).double().norm()
+                          / grad_batched.double().norm())}},
         "autotune": tune,
         "roofline": roofline,
         "kernels": kernels,

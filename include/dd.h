@@ -152,13 +152,17 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
 /* Uneven batches only: the 4 x 4 pixel blocks, each line's bins read as one
  * window of loads instead of one bin at a time. */
 #define DD_ADJ_BLOCK_WINDOW 4
-/* Uneven batches only: the 4 x 4 pixel blocks through per-view running
- * integrals of the sinogram along the detector (FP64 knots at the bin
- * edges): each pixel's value from a view is the difference of the running
- * integral at its two edges, found by one merge walk per 4-pixel run.
- * PROJ_T and BPROJ only (BPROJ_W, whose weight varies per pixel, runs
- * DD_ADJ_BLOCK_WINDOW).  Same operator, different rounding: not bit-equal
- * to the other variants. */
+/* The 4 x 4 pixel blocks through per-view running integrals of the sinogram
+ * along the detector (FP64 knots at the bin edges): each pixel's value from
+ * a view is the difference of the running integral at its two edges, found
+ * by one merge walk per 4-pixel run.  PROJ_T and BPROJ only (BPROJ_W, whose
+ * weight varies per pixel, keeps a walk).  In uneven batches; and in
+ * dd2d_run / planned runs at instance tile 1, where the views are split over
+ * workgroups (partial images added in order): a 512^2 x 720-view slice's
+ * ProjTrans 0.3 ms instead of the walk's 1.06 ms.  Same operator, different
+ * rounding: not bit-equal to the other variants (5e-8 from FP64 against the
+ * walks' 2-3e-7), so the automatic choice keeps the walks and this variant
+ * is selected explicitly or by the autotuner. */
 #define DD_ADJ_BLOCK_PREFIX 5
 /* Flag OR-ed into a variant, uneven batches only: the image -> sinogram ops
  * PROJ and BPROJ_T integrate each bin over each line as a difference of the
@@ -273,9 +277,11 @@ void dd_profile_enable(int on);
 int dd_profile_read(int kernel_class, double* total_ms, long long* launches);
 
 /* Default sinogram -> image kernel variant of dd2d_run and of planned runs
- * given variant -1 (DD_ADJ_*; -1 = automatic: DD_ADJ_QUAD for up to 4
- * instances per launch when that still gives 4 workgroups per compute unit,
- * DD_ADJ_PER_PIXEL otherwise). */
+ * given variant -1 (DD_ADJ_PER_PIXEL .. DD_ADJ_QUAD or DD_ADJ_BLOCK_PREFIX;
+ * -1 = automatic: DD_ADJ_QUAD for up to 4 instances per launch when that
+ * still gives 4 workgroups per compute unit, DD_ADJ_PER_PIXEL otherwise).
+ * DD_ADJ_BLOCK_PREFIX at a tile above 1 falls back to the automatic choice.
+ * dd2d_workspace_size follows the current setting. */
 void dd2d_set_adjoint_variant(int variant);
 
 /* Instance tile used per launch (1, 2, 4, 8 or 16 instances share one

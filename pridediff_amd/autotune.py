@@ -29,7 +29,7 @@ CACHE = os.environ.get("PRIDEDIFF_AMD_TUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pridediff_amd", "autotune.json"))
 
 
-SWEEP_VERSION = 8  # bumped when the set of swept parameters changes
+SWEEP_VERSION = 9  # bumped when the set of swept parameters changes
 
 _LIB_DIGEST = None
 
@@ -96,7 +96,8 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
                  variants=None, use_cache: bool = True, shapes: bool = True,
                  report: dict | None = None) -> BatchedProjector2d:
     """Fastest ``BatchedProjector2d`` for this projector, angle set and batch:
-    first the instance tile x adjoint kernel variant, then, for the winner,
+    first the instance tile x adjoint kernel variant (at tile 1 also the
+    running-sum adjoint), then, for the winner,
     the workgroup tile of each gather — forward (63-bin waves x views per
     workgroup; its lines are one of the 8 per-XCD chunks) and per-pixel
     adjoint (pixels per workgroup row)."""
@@ -118,7 +119,8 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
     for t in tiles:
         if t > 1 and t // 2 >= batch:
             continue
-        for var in variants:
+        # the running-sum adjoint runs at tile 1 only
+        for var in tuple(variants) + ((rt.ADJ_BLOCK_PREFIX,) if t == 1 else ()):
             bp = BatchedProjector2d(projector, angles, batch, tile=t, variant=var)
             with torch.no_grad():
                 results[(t, var)] = time_ms(lambda: (bp.plan.run(rt.PROJ, x),

@@ -53,11 +53,26 @@ int compute_units() {
   return cus;
 }
 
+bool known_variant(int v) {
+  return (v >= DD_ADJ_PER_PIXEL && v <= DD_ADJ_QUAD) || v == DD_ADJ_BLOCK_PREFIX;
+}
+
+// The automatic choice keeps the walks, so a slice run alone gives the bits
+// it gets in any batch; the running-sum adjoint (DD_ADJ_BLOCK_PREFIX, tile 1
+// only) is chosen explicitly or by the autotuner.
 int adjoint_variant(int requested, const Geo2& g) {
-  if (requested < DD_ADJ_PER_PIXEL || requested > DD_ADJ_QUAD) requested = g_adjoint_variant;
-  if (requested >= DD_ADJ_PER_PIXEL) return requested;
+  if (!known_variant(requested)) requested = g_adjoint_variant;
+  // the running sums are per instance: instance tile 1 only
+  if (requested == DD_ADJ_BLOCK_PREFIX && g.BT != 1) requested = -1;
+  if (known_variant(requested)) return requested;
   const long long quad_groups = 1LL * ((g.W + 31) / 32) * ((g.H + 31) / 32) * (g.Bp / g.BT);
   return (g.BT <= 4 && quad_groups >= 4LL * compute_units()) ? DD_ADJ_QUAD : DD_ADJ_PER_PIXEL;
+}
+
+// Whether op runs through the running-sum adjoint (its weights per pixel
+// keep BackProjWeighted on the walk).
+bool prefix_path(const Geo2& g, int op, int variant) {
+  return variant == DD_ADJ_BLOCK_PREFIX && g.BT == 1 && (op == DD_PROJ_T || op == DD_BPROJ);
 }
 
 // Pixel workgroup width of the per-pixel sinogram -> image gather (256
@@ -280,6 +295,144 @@ __global__ __launch_bounds__(T * T) void k_pixels_tile(
 }
 
 // ---------------------------------------------------------------------------
+// Variant DD_ADJ_BLOCK_PREFIX at instance tile 1: sinogram -> image through
+// running sums over the bins, with the views split over workgroups
+// ---------------------------------------------------------------------------
+// A single slice gives 64 x 64-pixel tiles for only 64 workgroups at 512^2,
+// so each tile's views are split into chunks: workgroup (tile, chunk,
+// instance) adds its chunk's views into a partial image, and k_sum_chunks
+// adds the partial images in chunk order.  Per view the same evaluation as
+// the batch path's running-sum adjoint (block_view_prefix over FP64 knots
+// staged in LDS).
+constexpr int kPrefQ = 4;                          // pixels per thread side
+constexpr int kPrefSide = 16;                      // threads per tile side
+constexpr int kPrefThreads = kPrefSide * kPrefSide;
+constexpr int kPrefTile = kPrefSide * kPrefQ;      // 64 x 64 pixels
+constexpr int kPrefKnots = kPrefThreads;           // knots staged per view
+
+__host__ __device__ inline int prefix_tiles(const Geo2& g) {
+  return ((g.W + kPrefTile - 1) / kPrefTile) * ((g.H + kPrefTile - 1) / kPrefTile);
+}
+
+// View chunks: about 1024 workgroups in all, at least 16 views per chunk.
+int prefix_chunks(const Geo2& g) {
+  const long long items = 1LL * prefix_tiles(g) * g.B;
+  long long c = (1024 + items - 1) / items;
+  c = std::min<long long>(c, std::max(1, g.V / 16));
+  return static_cast<int>(std::max<long long>(1, std::min<long long>(c, 64)));
+}
+
+// Knots of view v of instance b: one workgroup per (view, instance).
+template <bool FOLD_W>
+__global__ __launch_bounds__(kThreads) void k_prefix_sino(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const float* __restrict__ wfp, const float* __restrict__ sino,
+    double2* __restrict__ psino) {
+  __shared__ double lds[kThreads];
+  const int v = blockIdx.x, b = blockIdx.y;
+  prefix_sino_view<FOLD_W>(g, views, edges, wfp, sino + static_cast<size_t>(b) * g.V * g.D, v,
+                           psino + (static_cast<size_t>(b) * g.V + v) * (g.D + 1), lds);
+}
+
+// Knot range of every (view, 64 x 64 tile): kr[v * tiles + t].
+__global__ __launch_bounds__(kThreads) void k_prefix_ranges(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, int2* __restrict__ kr) {
+  const int tx = (g.W + kPrefTile - 1) / kPrefTile;
+  const int tiles = prefix_tiles(g);
+  const long long idx = 1LL * blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 1LL * tiles * g.V) return;
+  const int v = static_cast<int>(idx / tiles), t = static_cast<int>(idx % tiles);
+  const int c0 = (t % tx) * kPrefTile, r0 = (t / tx) * kPrefTile;
+  kr[idx] = knot_range(g, views[v], edges + static_cast<size_t>(v) * (g.D + 1),
+                       lines + static_cast<size_t>(v) * g.Lmax, r0, min(r0 + kPrefTile, g.H),
+                       c0, min(c0 + kPrefTile, g.W));
+}
+
+// Workgroup (tile, chunk, instance): views [chunk * vchunk, +vchunk).  Each
+// view's knots for the tile are staged in LDS (double-buffered: the next
+// view's copy is loaded while this one is walked, one barrier per view); a
+// view whose range exceeds the buffer is walked from global memory.
+template <int WM>
+__global__ __launch_bounds__(kPrefThreads)
+__attribute__((amdgpu_waves_per_eu(4))) void k_pixels_prefix(
+    Geo2 g, const ViewGeo* __restrict__ views, const LineGeo* __restrict__ lines,
+    const float* __restrict__ pint, const double2* __restrict__ psino,
+    const int2* __restrict__ kr, float* __restrict__ part, int vchunk) {
+  __shared__ double2 knots[2][kPrefKnots];
+  const int tid = static_cast<int>(threadIdx.x);
+  const int tx = (g.W + kPrefTile - 1) / kPrefTile;
+  const int tiles = prefix_tiles(g);
+  const int t = blockIdx.x, b = blockIdx.z;
+  const int v0 = blockIdx.y * vchunk, v1 = min(v0 + vchunk, g.V);
+  const int c0 = (t % tx) * kPrefTile + kPrefQ * (tid % kPrefSide);
+  const int r0 = (t / tx) * kPrefTile + kPrefQ * (tid / kPrefSide);
+  const bool active = c0 < g.W && r0 < g.H;
+  const int nr = min(kPrefQ, g.H - r0), nc = min(kPrefQ, g.W - c0);
+  const double2* __restrict__ tab = psino + static_cast<size_t>(b) * g.V * (g.D + 1);
+  const int2* __restrict__ rng = kr + t;  // view v at rng[v * tiles]
+  float acc[kPrefQ][kPrefQ];
+#pragma unroll
+  for (int dr = 0; dr < kPrefQ; ++dr)
+#pragma unroll
+    for (int dc = 0; dc < kPrefQ; ++dc) acc[dr][dc] = 0.0f;
+  if (v0 < v1) {
+    int2 r = rng[static_cast<size_t>(v0) * tiles];
+    bool staged = r.y - r.x + 1 <= kPrefKnots;
+    if (staged && tid <= r.y - r.x)
+      knots[v0 & 1][tid] = tab[static_cast<size_t>(v0) * (g.D + 1) + r.x + tid];
+    __syncthreads();
+    for (int v = v0; v < v1; ++v) {
+      // the next view's knot for this thread, in flight during this view's walks
+      const int vn = min(v + 1, v1 - 1);
+      const int2 rn = rng[static_cast<size_t>(vn) * tiles];
+      const bool staged_n = rn.y - rn.x + 1 <= kPrefKnots;
+      double2 kn = make_double2(0.0, 0.0);
+      const bool mine = v + 1 < v1 && staged_n && tid <= rn.y - rn.x;
+      if (mine) kn = tab[static_cast<size_t>(vn) * (g.D + 1) + rn.x + tid];
+      if (active) {
+        const ViewGeo vg = views[v];
+        const LineGeo* __restrict__ lv = lines + static_cast<size_t>(v) * g.Lmax;
+        const float* __restrict__ pv = pint + static_cast<size_t>(v) * g.Lmax;
+        if (staged)
+          block_view_prefix<WM, kPrefQ>(
+              g, vg, lv, pv, (const __attribute__((address_space(3))) double*)(knots[v & 1]),
+              r.x, r.y, r0, c0, nr, nc, acc);
+        else
+          block_view_prefix<WM, kPrefQ>(
+              g, vg, lv, pv,
+              reinterpret_cast<const double*>(tab + static_cast<size_t>(v) * (g.D + 1)), 0,
+              g.D, r0, c0, nr, nc, acc);
+      }
+      if (mine) knots[(v + 1) & 1][tid] = kn;
+      __syncthreads();
+      r = rn;
+      staged = staged_n;
+    }
+  }
+  if (active) {
+    float* __restrict__ o =
+        part + (static_cast<size_t>(blockIdx.y) * g.B + b) * g.H * g.W;
+#pragma unroll
+    for (int dr = 0; dr < kPrefQ; ++dr)
+#pragma unroll
+      for (int dc = 0; dc < kPrefQ; ++dc)
+        if (dr < nr && dc < nc) o[static_cast<size_t>(r0 + dr) * g.W + c0 + dc] = acc[dr][dc];
+  }
+}
+
+// out = epilogue(sum of the chunks' partial images, in chunk order).
+__global__ __launch_bounds__(kThreads) void k_sum_chunks(
+    Geo2 g, const float* __restrict__ part, int nchunk, float* __restrict__ out, Epi ep) {
+  const size_t n = static_cast<size_t>(g.B) * g.H * g.W;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = part[i];
+  for (int c = 1; c < nchunk; ++c) s += part[static_cast<size_t>(c) * n + i];
+  out[i] = epi_value(ep, s, i);
+}
+
+// ---------------------------------------------------------------------------
 // system matrix (COO, FP weights): count pass and fill pass of one walk
 // ---------------------------------------------------------------------------
 // Entry (v*D + detector, row*W + col) = overlap on the axis * FP weight,
@@ -335,6 +488,7 @@ __global__ __launch_bounds__(kThreads) void k_sysmatrix(
 struct Layout {
   size_t views, edges, wfp, lines, pint, geo_total;  // geometry tables
   size_t buf0, buf1, part, data_total;                // data buffers
+  size_t psino, krange, ppart;                        // running-sum adjoint
 };
 
 constexpr size_t kAlign = 256;
@@ -351,7 +505,9 @@ bool to_sino(int op) {
   return op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW;
 }
 
-Layout make_layout(const Geo2& g, int op) {
+// prefix: also the running-sum adjoint's knots [B][V][D+1] (16 bytes each),
+// knot ranges [V][tiles] and partial images [chunks][B][H][W].
+Layout make_layout(const Geo2& g, int op, bool prefix = false) {
   Layout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -373,9 +529,20 @@ Layout make_layout(const Geo2& g, int op) {
   } else {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * g.D * g.Bp);
     L.buf1 = L.buf0;
+    if (prefix) {
+      L.psino = take(16 * static_cast<size_t>(g.B) * g.V * (g.D + 1));
+      L.krange = take(8 * static_cast<size_t>(g.V) * prefix_tiles(g));
+      L.ppart = take(sizeof(float) * static_cast<size_t>(prefix_chunks(g)) * g.B * g.H * g.W);
+    }
   }
   L.data_total = off;
   return L;
+}
+
+// Planned workspaces hold the running-sum adjoint's buffers whenever it can
+// run (instance tile 1), so any variant may be chosen after planning.
+bool planned_prefix(const Geo2& g, int op) {
+  return g.BT == 1 && (op == DD_PROJ_T || op == DD_BPROJ);
 }
 
 bool fill_geo(Geo2* out, int scan, const dd_geom2d* g, int batch, int views,
@@ -484,6 +651,33 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       hipLaunchKernelGGL((k_bins_reduce<W_BPW>), rgrid, dim3(kThreads), 0, st, g, G.views,
                          G.wfp, part, output, ep);
     }
+  } else if (BT == 1 && prefix_path(g, op, variant)) {
+    auto* psino = reinterpret_cast<double2*>(ws + L.psino);
+    auto* kr = reinterpret_cast<int2*>(ws + L.krange);
+    auto* pp = reinterpret_cast<float*>(ws + L.ppart);
+    const int tiles = prefix_tiles(g);
+    const int vchunk = (g.V + prefix_chunks(g) - 1) / prefix_chunks(g);
+    const int nchunk = (g.V + vchunk - 1) / vchunk;
+    const long long nr = 1LL * tiles * g.V;
+    const size_t npix = static_cast<size_t>(g.B) * g.H * g.W;
+    ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
+    if (op == DD_PROJ_T)
+      hipLaunchKernelGGL((k_prefix_sino<true>), dim3(g.V, g.B), dim3(kThreads), 0, st, g,
+                         G.views, G.edges, G.wfp, input, psino);
+    else
+      hipLaunchKernelGGL((k_prefix_sino<false>), dim3(g.V, g.B), dim3(kThreads), 0, st, g,
+                         G.views, G.edges, G.wfp, input, psino);
+    hipLaunchKernelGGL(k_prefix_ranges, dim3(static_cast<unsigned>((nr + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, g, G.views, G.edges, G.lines, kr);
+    const dim3 grid(tiles, nchunk, g.B);
+    if (op == DD_PROJ_T)
+      hipLaunchKernelGGL((k_pixels_prefix<W_FP>), grid, dim3(kPrefThreads), 0, st, g, G.views,
+                         G.lines, G.pint, psino, kr, pp, vchunk);
+    else
+      hipLaunchKernelGGL((k_pixels_prefix<W_BP>), grid, dim3(kPrefThreads), 0, st, g, G.views,
+                         G.lines, G.pint, psino, kr, pp, vchunk);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(static_cast<unsigned>((npix + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, g, pp, nchunk, output, ep);
   } else {
     dim3 sgrid((g.D + kThreads - 1) / kThreads, g.V, ntile);
     if (op == DD_PROJ_T)
@@ -621,7 +815,7 @@ extern "C" {
 const char* dd_version(void) { return "libdd 0.1 (gfx950)"; }
 
 void dd2d_set_adjoint_variant(int variant) {
-  g_adjoint_variant = (variant >= DD_ADJ_PER_PIXEL && variant <= DD_ADJ_QUAD) ? variant : -1;
+  g_adjoint_variant = known_variant(variant) ? variant : -1;
 }
 
 void dd2d_set_instance_tile(int tile) {
@@ -632,7 +826,7 @@ size_t dd2d_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
                            int views) {
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return 0;
-  const Layout L = make_layout(q, op);
+  const Layout L = make_layout(q, op, prefix_path(q, op, adjoint_variant(-1, q)));
   return L.geo_total + L.data_total;
 }
 
@@ -644,13 +838,13 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   if (!input || !output || !angles) return DD_BAD_ARG;
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
-  const Layout L = make_layout(q, op);
+  const int var = adjoint_variant(-1, q);
+  const Layout L = make_layout(q, op, prefix_path(q, op, var));
   if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
   launch_setup(q, angles, w, L, st);
-  return run_tiles(q, op, adjoint_variant(-1, q), input, output, geo_ptrs(w, L),
-                   w + L.geo_total, L, st);
+  return run_tiles(q, op, var, input, output, geo_ptrs(w, L), w + L.geo_total, L, st);
 }
 
 dd_status dd2d_run_noalloc(int scan, int op, const dd_geom2d* g, const float* input,
@@ -661,7 +855,8 @@ dd_status dd2d_run_noalloc(int scan, int op, const dd_geom2d* g, const float* in
   if (!input || !output || !angles) return DD_BAD_ARG;
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
-  const Layout L = make_layout(q, op);
+  const int var = adjoint_variant(-1, q);
+  const Layout L = make_layout(q, op, prefix_path(q, op, var));
   if (L.geo_total + L.data_total > DD_NOALLOC_SCRATCH_BYTES) return DD_TOO_LARGE;
   char* w = static_cast<char*>(scratch_ptr());
   if (!w) return DD_LAUNCH_FAIL;
@@ -669,8 +864,7 @@ dd_status dd2d_run_noalloc(int scan, int op, const dd_geom2d* g, const float* in
   // own launches before it is read, so nothing needs clearing between calls
   auto st = static_cast<hipStream_t>(stream);
   launch_setup(q, angles, w, L, st);
-  return run_tiles(q, op, adjoint_variant(-1, q), input, output, geo_ptrs(w, L),
-                   w + L.geo_total, L, st);
+  return run_tiles(q, op, var, input, output, geo_ptrs(w, L), w + L.geo_total, L, st);
 }
 
 dd_status dd2d_run_views(int scan, int op, const dd_geom2d* g, const float* input,
@@ -733,7 +927,7 @@ size_t dd2d_planned_workspace_size(int scan, int op, const dd_geom2d* g,
                                    int batch, int views, int tile) {
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return 0;
-  return make_layout(q, op).data_total;
+  return make_layout(q, op, planned_prefix(q, op)).data_total;
 }
 
 dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
@@ -745,7 +939,7 @@ dd_status dd2d_run_planned(int scan, int op, const dd_geom2d* g,
   if (!geo || !input || !output) return DD_BAD_ARG;
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return DD_BAD_ARG;
-  const Layout L = make_layout(q, op);
+  const Layout L = make_layout(q, op, planned_prefix(q, op));
   if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
   return run_tiles(q, op, adjoint_variant(variant, q), input, output, geo_ptrs(geo, L),
                    static_cast<char*>(ws), L, static_cast<hipStream_t>(stream));
@@ -764,7 +958,7 @@ dd_status dd2d_run_planned_ep(int scan, int op, const dd_geom2d* g,
   if (!make_shape(shape, &sh)) return DD_BAD_ARG;
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, valid_tile(tile, batch))) return DD_BAD_ARG;
-  const Layout L = make_layout(q, op);
+  const Layout L = make_layout(q, op, planned_prefix(q, op));
   if (!ws || ws_bytes < L.data_total) return DD_NO_WORKSPACE;
   Epi e;
   if (ep) {

@@ -341,7 +341,8 @@ class Plan2d {
         variant_(static_cast<int>(variant)),
         shape_{static_cast<int>(fwd_waves), static_cast<int>(fwd_views),
                static_cast<int>(pix_width)} {
-    TORCH_CHECK(variant >= -1 && variant <= DD_ADJ_QUAD, "Plan2d: unknown kernel variant");
+    TORCH_CHECK((variant >= -1 && variant <= DD_ADJ_QUAD) || variant == DD_ADJ_BLOCK_PREFIX,
+                "Plan2d: unknown kernel variant");
     TORCH_CHECK(fwd_waves == 0 || fwd_waves == 1 || fwd_waves == 2 || fwd_waves == 4,
                 "Plan2d: fwd_waves must be 1, 2 or 4 (0 = default)");
     TORCH_CHECK(fwd_views >= 0 && fwd_views <= 8, "Plan2d: fwd_views must be 1..8 (0 = default)");

@@ -1,0 +1,170 @@
+"""Running-sum adjoint of single calls (needs an MI355X).
+
+At instance tile 1, DD_ADJ_BLOCK_PREFIX runs ProjTrans and BackProj of
+dd2d_run / planned runs through per-view running integrals of the sinogram
+(FP64 knots), with each 64 x 64 pixel tile's views split over workgroups and
+the partial images added in order; it is chosen explicitly (or by the
+autotuner), the automatic choice keeping the walks.  Same operator as the
+walks, different rounding, so it is held to the FP64 oracle and to the walks
+by tolerance:
+
+* ProjTrans and BackProj vs the FP64 oracle <= 1e-6 relative L2 on para /
+  fan-flat / fan-arc with shifted and rectangular images, bins narrower and
+  wider than a pixel, and inputs on a 1e3 offset (running sums ~1e3 times
+  the values they are differences of); single calls and a batch of 3 at a
+  forced tile of 1;
+* dot products against the forward <= 1e-6;
+* BackProjWeighted keeps its walk: equal bit for bit under the variant;
+* a 512^2 x 720-view slice within 1e-6 of FP64 and of the walk, and the
+  default choice there equal bit for bit to the walk;
+* a planned run equals the single call bit for bit, its epilogue
+  (x - A^T y in the kernel's write) equals the composition bit for bit, and
+  a captured graph replays it bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pridediff_amd.phantom import ellipses2d
+
+pytestmark = pytest.mark.gpu
+
+OPNAMES = {"proj": "Proj", "proj_t": "ProjTrans", "bproj": "BackProj",
+           "bproj_w": "BackProjWeighted"}
+GEOMNAMES = {"para": "ParaDis2d", "flat": "FanFlatDis2d", "arc": "FanArcDis2d"}
+
+# (W, H, D, pixel, det scale, pixel shift, bin shift, views)
+CASES = ((96, 96, 141, 1.1, 1.0, (0.0, 0.0), 0.0, 120),
+         (120, 87, 300, 1.4, 0.25, (3.1, -2.2), 0.7, 97),
+         (70, 70, 25, 1.4, 6.0, (0.0, 1.5), -2.0, 64),
+         (130, 140, 185, 0.7, 1.2858 / 0.7 * 0.7, (0.0, 0.0), 0.0, 150))
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from pridediff_amd import load_radontorch
+    return load_radontorch()
+
+
+def _geom(scan, case):
+    w, h, d, px, ds, sh, bs, v = case
+    if scan == "para":
+        g = O.Geom2D(w, h, d, px, px, 1.2858 * ds, 1.0, 1.0, sh[0], sh[1], bs)
+    elif scan == "flat":
+        g = O.Geom2D(w, h, d, px, px, 1.2858 * ds, 595.0, 1085.6, sh[0], sh[1], bs)
+    else:
+        g = O.Geom2D(w, h, d, px, px, 1.2858 * ds / 1085.6, 595.0, 1085.6, sh[0], sh[1],
+                     bs / 1085.6)
+    ang = (np.arange(v) * (2 * np.pi / v) + 0.05).astype(np.float32)
+    return g, ang
+
+
+def _call(rt, scan, op, x, ang, g):
+    fn = getattr(rt, OPNAMES[op] + GEOMNAMES[scan])
+    xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32))[:, None].cuda()
+    out = fn(xt, torch.tensor(g.options(), dtype=torch.float32),
+             torch.from_numpy(ang).cuda())
+    torch.cuda.synchronize()
+    return out[:, 0].cpu().numpy()
+
+
+def _with_variant(rt, var, fn):
+    rt.set_adjoint_variant(var)
+    try:
+        return fn()
+    finally:
+        rt.set_adjoint_variant(-1)
+
+
+@pytest.mark.parametrize("scan", ["para", "flat", "arc"])
+@pytest.mark.parametrize("offset", [0.0, 1000.0])
+def test_single_call_running_sums_match_fp64(rt, scan, offset):
+    rng = np.random.default_rng(21)
+    try:
+        for batch, tile in ((1, 0), (3, 1)):
+            rt.set_instance_tile(tile)
+            for i, case in enumerate(CASES):
+                g, ang = _geom(scan, case)
+                y = rng.random((batch, len(ang), g.D)).astype(np.float32) + np.float32(offset)
+                for op in ("proj_t", "bproj"):
+                    got = _with_variant(rt, rt.ADJ_BLOCK_PREFIX,
+                                        lambda: _call(rt, scan, op, y, ang, g))
+                    walk = _with_variant(rt, rt.ADJ_PER_PIXEL,
+                                         lambda: _call(rt, scan, op, y, ang, g))
+                    want = O.op2d(scan, op, y.astype(np.float64), ang, g, "f64")
+                    r, rw = O.rel_l2(got, want), O.rel_l2(walk, want)
+                    print(f"{scan} offset {offset} batch {batch} case {i} {op}: running sums "
+                          f"{r:.2e}, walk {rw:.2e}", flush=True)
+                    assert r <= 1e-6, (scan, offset, batch, i, op, r, rw)
+    finally:
+        rt.set_instance_tile(0)
+
+
+@pytest.mark.parametrize("scan", ["para", "flat", "arc"])
+def test_single_call_running_sums_dot_products(rt, scan):
+    rng = np.random.default_rng(22)
+    for i, case in enumerate(CASES):
+        g, ang = _geom(scan, case)
+        x = rng.random((1, g.H, g.W)).astype(np.float32)
+        y = rng.random((1, len(ang), g.D)).astype(np.float32)
+        ax = _call(rt, scan, "proj", x, ang, g).astype(np.float64)
+        aty = _with_variant(rt, rt.ADJ_BLOCK_PREFIX,
+                            lambda: _call(rt, scan, "proj_t", y, ang, g)).astype(np.float64)
+        lhs, rhs = float(np.sum(ax * y)), float(np.sum(x * aty))
+        assert abs(lhs - rhs) / abs(rhs) <= 1e-6, (scan, i, lhs, rhs)
+
+
+@pytest.mark.parametrize("scan", ["flat", "arc"])
+def test_weighted_backprojection_keeps_its_walk(rt, scan):
+    rng = np.random.default_rng(23)
+    g, ang = _geom(scan, CASES[1])
+    y = rng.random((1, len(ang), g.D)).astype(np.float32)
+    a = _with_variant(rt, rt.ADJ_BLOCK_PREFIX, lambda: _call(rt, scan, "bproj_w", y, ang, g))
+    b = _with_variant(rt, rt.ADJ_PER_PIXEL, lambda: _call(rt, scan, "bproj_w", y, ang, g))
+    assert np.array_equal(a, b)
+
+
+def test_full_size_slice_and_default(rt):
+    """A 512^2 fan-arc slice of 720 views: the running sums within 1e-6 of
+    FP64 and of the walk; the default (automatic) choice keeps the walk, so
+    a slice run alone gets the bits it gets in a batch."""
+    rng = np.random.default_rng(24)
+    g = O.Geom2D(512, 512, 736, 0.7, 0.7, 1.2858 / 1085.6, 595.0, 1085.6)
+    ang = (np.arange(720) * (2 * np.pi / 720)).astype(np.float32)
+    y = rng.random((1, 720, 736)).astype(np.float32)
+    pref = _with_variant(rt, rt.ADJ_BLOCK_PREFIX, lambda: _call(rt, "arc", "proj_t", y, ang, g))
+    walk = _with_variant(rt, rt.ADJ_PER_PIXEL, lambda: _call(rt, "arc", "proj_t", y, ang, g))
+    auto = _call(rt, "arc", "proj_t", y, ang, g)
+    want = O.op2d("arc", "proj_t", y.astype(np.float64), ang, g, "f64")
+    r, rw = O.rel_l2(pref, want), O.rel_l2(walk, want)
+    print(f"512^2 x 720 proj_t: running sums {r:.2e}, walk {rw:.2e}", flush=True)
+    assert r <= 1e-6
+    assert O.rel_l2(pref, walk) <= 1e-6
+    assert np.array_equal(auto, walk)
+
+
+def test_planned_epilogue_and_graph_replay(rt):
+    from pridediff_amd.batched import GraphStep
+    from pridediff_amd.deeprecon.torch.projector import projector2d
+    P = projector2d((120, 87), 300, 1.4, 1.2858 * 0.25, 595.0, 1085.6, pixshift=(3.1, -2.2),
+                    binshift=0.7, scan_type="arc").cuda()
+    v = 97
+    ang = torch.from_numpy((np.arange(v) * (2 * np.pi / v) + 0.05).astype(np.float32)).cuda()
+    plan = rt.Plan2d(rt.SCAN[P.scan_type], P.host_options(), ang, 1, 1, rt.ADJ_BLOCK_PREFIX)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    y = torch.rand(1, 1, v, 300, device="cuda", generator=gen)
+    x = torch.from_numpy(ellipses2d(120, 1, seed=9)[:, :87, :]).cuda()[:, None].contiguous()
+    with torch.no_grad():
+        planned = plan.run(rt.PROJ_T, y)
+        single = _with_variant(rt, rt.ADJ_BLOCK_PREFIX, lambda: P.projection_t(y, ang))
+        assert torch.equal(planned, single)
+        fused = plan.run_ep(rt.PROJ_T, y, c=x, alpha=-1.0, beta=1.0)
+        assert torch.equal(fused, x - planned)
+        step = GraphStep(lambda a, b: plan.run_ep(rt.PROJ_T, a, c=b, alpha=-1.0, beta=1.0),
+                         y.clone(), x.clone())
+        for i in range(3):
+            got = step.replay(y, x).clone()
+            torch.cuda.synchronize()
+            _ = torch.randn(100, device="cuda").sum()
+            assert torch.equal(got, fused), f"replay {i}"
