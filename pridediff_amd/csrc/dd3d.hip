@@ -42,7 +42,8 @@ constexpr int kThreads3 = 256;
 //    tiles), two chunks per workgroup.  Consecutive workgroups are
 //    consecutive views of one block, so the blocks in flight sweep the whole
 //    volume copy; best while that copy stays in the 256 MB Infinity Cache
-//    (C3: 17.7 ms; 17.5 / 17.3 ms at two / one chunk against 4).
+//    (C3 17.5 ms; with one / four chunks per workgroup 17.3 / 17.7 ms, C4
+//    77.4 / 77.6 against 75.4 at two).
 //  * columns first: grid (column blocks, views, row chunks x instance tiles),
 //    one chunk per workgroup.  The workgroups in flight are the column blocks
 //    of a few hundred consecutive views for one row chunk: a band of slices,
