@@ -4,8 +4,9 @@ called through the C ABI (needs an MI355X).
 The no-allocation call takes its scratch from a fixed per-device buffer in
 the library; its results must equal dd2d_run's (caller workspace) bit for
 bit, on two calls in a row where the second is smaller (the buffer keeps the
-first call's contents), for every op of every 2D geometry.  Problems larger
-than the buffer are rejected with DD_TOO_LARGE before any device work.
+first call's contents), for every op of every 2D geometry, and with the
+single-call running-sum adjoint selected.  Problems larger than the buffer
+are rejected with DD_TOO_LARGE before any device work.
 """
 import ctypes
 import os
@@ -93,6 +94,29 @@ def test_noalloc_equals_workspace_call_twice(lib, scan):
             assert ws_bytes <= 64 << 20
             assert not torch.isnan(a).any()
             assert torch.equal(a, b), (scan, op, k, (a - b).abs().max().item())
+
+
+@pytest.mark.parametrize("scan", [0, 2])
+def test_noalloc_with_running_sum_adjoint(lib, scan):
+    """With the single-call running-sum adjoint selected (ProjTrans and
+    BackProj at instance tile 1: knots, knot ranges and per-chunk partial
+    images in the scratch too), the same two calls agree bit for bit."""
+    lib.dd2d_set_adjoint_variant.argtypes = [ctypes.c_int]
+    lib.dd2d_set_adjoint_variant(5)  # DD_ADJ_BLOCK_PREFIX
+    try:
+        for op in (1, 2):
+            for k, (n, d, views, batch, scale) in enumerate(((512, 736, 720, 1, 1.0),
+                                                             (96, 141, 90, 1, 4.0))):
+                g = _geom(scan, n, d, scale)
+                print(f"running sums: scan {scan} op {op} call {k}: {n}x{n}, {views} views",
+                      flush=True)
+                a, b, ws_bytes = _run_both(lib, scan, op, g, batch, views,
+                                           seed=1000 + 100 * scan + 10 * op + k)
+                assert ws_bytes <= 64 << 20
+                assert not torch.isnan(a).any()
+                assert torch.equal(a, b), (scan, op, k, (a - b).abs().max().item())
+    finally:
+        lib.dd2d_set_adjoint_variant(-1)
 
 
 def test_noalloc_rejects_problems_beyond_its_buffer(lib):
