@@ -803,9 +803,10 @@ def main():
 
     grad_plain = torch.cat(plain_step())
     t_plain = timed(torch, plain_step, args.plain_steps, 1)
-    # the same calls with the single-call running-sum adjoint (opt-in variant:
-    # the default keeps the walks, bit-identical to the batched path)
-    rt.set_adjoint_variant(rt.ADJ_BLOCK_PREFIX)
+    # the same calls through single-call running sums in both directions
+    # (opt-in variant: the default keeps the walks, bit-identical to the
+    # batched path)
+    rt.set_adjoint_variant(rt.ADJ_BLOCK_PREFIX | rt.FWD_PREFIX)
     try:
         grad_plain_rs = torch.cat(plain_step())
         t_plain_rs = timed(torch, plain_step, args.plain_steps, 1)
@@ -897,7 +898,7 @@ def main():
                    "pixel_mm": PIX, "det_mm": DET, "sid_mm": SID, "sdd_mm": SDD},
         "plain": {"ms_per_step": t_plain * 1e3, "value": samples / t_plain,
                   "batched_speedup_per_slice": t_plain / t_batched,
-                  "running_sum_adjoint": {
+                  "running_sums": {
                       "ms_per_step": t_plain_rs * 1e3,
                       "batched_speedup_per_slice": t_plain_rs / t_batched,
                       "grad_rel_l2_vs_batched": float(

@@ -53,8 +53,12 @@ int compute_units() {
   return cus;
 }
 
+// A variant is a sinogram -> image kernel (DD_ADJ_*), optionally OR-ed with
+// DD_FWD_PREFIX (image -> sinogram through running sums, tile 1).
 bool known_variant(int v) {
-  return (v >= DD_ADJ_PER_PIXEL && v <= DD_ADJ_QUAD) || v == DD_ADJ_BLOCK_PREFIX;
+  if (v < 0) return false;
+  const int adj = v & ~DD_FWD_PREFIX;
+  return (adj >= DD_ADJ_PER_PIXEL && adj <= DD_ADJ_QUAD) || adj == DD_ADJ_BLOCK_PREFIX;
 }
 
 // The automatic choice keeps the walks, so a slice run alone gives the bits
@@ -63,16 +67,35 @@ bool known_variant(int v) {
 int adjoint_variant(int requested, const Geo2& g) {
   if (!known_variant(requested)) requested = g_adjoint_variant;
   // the running sums are per instance: instance tile 1 only
-  if (requested == DD_ADJ_BLOCK_PREFIX && g.BT != 1) requested = -1;
-  if (known_variant(requested)) return requested;
-  const long long quad_groups = 1LL * ((g.W + 31) / 32) * ((g.H + 31) / 32) * (g.Bp / g.BT);
-  return (g.BT <= 4 && quad_groups >= 4LL * compute_units()) ? DD_ADJ_QUAD : DD_ADJ_PER_PIXEL;
+  int flags = (requested >= 0 && g.BT == 1) ? (requested & DD_FWD_PREFIX) : 0;
+  int adj = requested >= 0 ? (requested & ~DD_FWD_PREFIX) : -1;
+  if (adj == DD_ADJ_BLOCK_PREFIX && g.BT != 1) adj = -1;
+  if (adj < 0) {
+    const long long quad_groups = 1LL * ((g.W + 31) / 32) * ((g.H + 31) / 32) * (g.Bp / g.BT);
+    adj = (g.BT <= 4 && quad_groups >= 4LL * compute_units()) ? DD_ADJ_QUAD : DD_ADJ_PER_PIXEL;
+  }
+  return adj | flags;
 }
 
 // Whether op runs through the running-sum adjoint (its weights per pixel
 // keep BackProjWeighted on the walk).
 bool prefix_path(const Geo2& g, int op, int variant) {
-  return variant == DD_ADJ_BLOCK_PREFIX && g.BT == 1 && (op == DD_PROJ_T || op == DD_BPROJ);
+  return (variant & ~DD_FWD_PREFIX) == DD_ADJ_BLOCK_PREFIX && g.BT == 1 &&
+         (op == DD_PROJ_T || op == DD_BPROJ);
+}
+
+// Whether op runs through the running-sum forward (BackProjTransWeighted
+// keeps the walk).
+bool fwd_prefix_path(const Geo2& g, int op, int variant) {
+  return (variant & DD_FWD_PREFIX) != 0 && g.BT == 1 && (op == DD_PROJ || op == DD_BPROJ_T);
+}
+
+// Running-sum buffers a layout holds.
+enum PrefixRegions { kPrefNone = 0, kPrefAdjoint = 1, kPrefForward = 2 };
+
+int prefix_regions(const Geo2& g, int op, int variant) {
+  return (prefix_path(g, op, variant) ? kPrefAdjoint : 0) |
+         (fwd_prefix_path(g, op, variant) ? kPrefForward : 0);
 }
 
 // Pixel workgroup width of the per-pixel sinogram -> image gather (256
@@ -421,6 +444,57 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_pixels_prefix(
   }
 }
 
+// Running-sum forward at instance tile 1 (DD_FWD_PREFIX): per (line,
+// instance) the line's running pixel sums as FP32 (hi, lo) pairs of an FP64
+// scan, {hi, lo, x, 0} per pixel edge; the H rows first, then the W columns
+// read bottom-up.
+__global__ __launch_bounds__(kThreads) void k_prefix_lines(
+    Geo2 g, const float* __restrict__ img, float4* __restrict__ prow,
+    float4* __restrict__ pcol) {
+  __shared__ double lds[kThreads];
+  const int line = blockIdx.x, b = blockIdx.y;
+  const int H = g.H, W = g.W;
+  const float* __restrict__ x = img + static_cast<size_t>(b) * H * W;
+  if (line < H)
+    prefix_line(x + static_cast<size_t>(line) * W, 1, W,
+                prow + (static_cast<size_t>(b) * H + line) * prefix_entries(W), lds);
+  else
+    prefix_line(x + static_cast<size_t>(H - 1) * W + (line - H), -static_cast<long long>(W), H,
+                pcol + (static_cast<size_t>(b) * W + line - H) * prefix_entries(H), lds);
+}
+
+// One wave per (line chunk, block of kBinsPerWave bins, view, instance),
+// chunk fastest: each bin's integral over each line of the chunk as a
+// difference of the line's running sums (bins_accumulate_prefix), into the
+// chunk's partial sinogram, which k_bins_reduce adds in chunk order.
+template <int WM>
+__global__ __launch_bounds__(64) void k_bins_prefix(
+    Geo2 g, const ViewGeo* __restrict__ views, const double* __restrict__ edges,
+    const LineGeo* __restrict__ lines, const float* __restrict__ pint,
+    const float4* __restrict__ prow, const float4* __restrict__ pcol,
+    float* __restrict__ part) {
+  const int kb_n = (g.D + kBinsPerWave - 1) / kBinsPerWave;
+  const int chunk = blockIdx.x % kSplit;
+  int rest = blockIdx.x / kSplit;
+  const int lane = static_cast<int>(threadIdx.x);
+  const int k = (rest % kb_n) * kBinsPerWave + lane;
+  rest /= kb_n;
+  const int v = rest % g.V, b = rest / g.V;
+  // the whole wave works (the upper edge comes from the next lane); lanes
+  // past the last bin use clamped edges and write nothing
+  const ViewGeo vg = views[v];
+  const int L = vg.xaxis ? g.H : g.W;
+  const int N = vg.xaxis ? g.W : g.H;
+  const double e1 = edges[static_cast<size_t>(v) * (g.D + 1) + min(k, g.D)];
+  const float4* __restrict__ tab =
+      (vg.xaxis ? prow : pcol) + static_cast<size_t>(b) * L * prefix_entries(N);
+  const float acc = bins_accumulate_prefix<WM>(
+      e1, lines + static_cast<size_t>(v) * g.Lmax, pint + static_cast<size_t>(v) * g.Lmax, tab,
+      bp_const(g), L * chunk / kSplit, L * (chunk + 1) / kSplit, N);
+  if (lane == 63 || k >= g.D) return;
+  part[((static_cast<size_t>(chunk) * g.Bp + b) * g.V + v) * g.D + k] = acc;
+}
+
 // out = epilogue(sum of the chunks' partial images, in chunk order).
 __global__ __launch_bounds__(kThreads) void k_sum_chunks(
     Geo2 g, const float* __restrict__ part, int nchunk, float* __restrict__ out, Epi ep) {
@@ -489,6 +563,7 @@ struct Layout {
   size_t views, edges, wfp, lines, pint, geo_total;  // geometry tables
   size_t buf0, buf1, part, data_total;                // data buffers
   size_t psino, krange, ppart;                        // running-sum adjoint
+  size_t prow, pcol;                                  // running-sum forward
 };
 
 constexpr size_t kAlign = 256;
@@ -505,9 +580,11 @@ bool to_sino(int op) {
   return op == DD_PROJ || op == DD_BPROJ_T || op == DD_BPROJ_TW;
 }
 
-// prefix: also the running-sum adjoint's knots [B][V][D+1] (16 bytes each),
-// knot ranges [V][tiles] and partial images [chunks][B][H][W].
-Layout make_layout(const Geo2& g, int op, bool prefix = false) {
+// prefix (PrefixRegions): also the running-sum adjoint's knots [B][V][D+1]
+// (16 bytes each), knot ranges [V][tiles] and partial images
+// [chunks][B][H][W]; or the running-sum forward's line tables, rows
+// [B][H][W+1] and columns [B][W][H+1] of 16-byte entries.
+Layout make_layout(const Geo2& g, int op, int prefix = kPrefNone) {
   Layout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -526,10 +603,14 @@ Layout make_layout(const Geo2& g, int op, bool prefix = false) {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
     L.buf1 = take(sizeof(float) * static_cast<size_t>(g.H) * g.W * g.Bp);
     L.part = take(sizeof(float) * static_cast<size_t>(kSplit) * g.Bp * g.V * g.D);
+    if (prefix & kPrefForward) {
+      L.prow = take(16 * static_cast<size_t>(g.B) * g.H * prefix_entries(g.W));
+      L.pcol = take(16 * static_cast<size_t>(g.B) * g.W * prefix_entries(g.H));
+    }
   } else {
     L.buf0 = take(sizeof(float) * static_cast<size_t>(g.V) * g.D * g.Bp);
     L.buf1 = L.buf0;
-    if (prefix) {
+    if (prefix & kPrefAdjoint) {
       L.psino = take(16 * static_cast<size_t>(g.B) * g.V * (g.D + 1));
       L.krange = take(8 * static_cast<size_t>(g.V) * prefix_tiles(g));
       L.ppart = take(sizeof(float) * static_cast<size_t>(prefix_chunks(g)) * g.B * g.H * g.W);
@@ -539,10 +620,13 @@ Layout make_layout(const Geo2& g, int op, bool prefix = false) {
   return L;
 }
 
-// Planned workspaces hold the running-sum adjoint's buffers whenever it can
-// run (instance tile 1), so any variant may be chosen after planning.
-bool planned_prefix(const Geo2& g, int op) {
-  return g.BT == 1 && (op == DD_PROJ_T || op == DD_BPROJ);
+// Planned workspaces hold the running-sum buffers whenever they can be used
+// (instance tile 1), so any variant may be chosen after planning.
+int planned_prefix(const Geo2& g, int op) {
+  if (g.BT != 1) return kPrefNone;
+  if (op == DD_PROJ_T || op == DD_BPROJ) return kPrefAdjoint;
+  if (op == DD_PROJ || op == DD_BPROJ_T) return kPrefForward;
+  return kPrefNone;
 }
 
 bool fill_geo(Geo2* out, int scan, const dd_geom2d* g, int batch, int views,
@@ -623,7 +707,29 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
   auto* buf0 = reinterpret_cast<float*>(ws + L.buf0);
   auto* buf1 = reinterpret_cast<float*>(ws + L.buf1);
   const int ntile = g.Bp / BT;
-  if (to_sino(op)) {
+  const int adj = variant & ~DD_FWD_PREFIX;
+  if (BT == 1 && fwd_prefix_path(g, op, variant)) {
+    auto* prow = reinterpret_cast<float4*>(ws + L.prow);
+    auto* pcol = reinterpret_cast<float4*>(ws + L.pcol);
+    auto* part = reinterpret_cast<float*>(ws + L.part);
+    const int kb_n = (g.D + kBinsPerWave - 1) / kBinsPerWave;
+    const long long items = 1LL * kSplit * kb_n * g.V * g.B;
+    dim3 rgrid(((g.D + kThreads - 1) / kThreads) * g.V * g.B);
+    ProfileScope prof(DD_KERNEL_IMAGE_TO_SINO, st);
+    hipLaunchKernelGGL(k_prefix_lines, dim3(g.H + g.W, g.B), dim3(kThreads), 0, st, g, input,
+                       prow, pcol);
+    if (op == DD_PROJ) {
+      hipLaunchKernelGGL((k_bins_prefix<W_FP>), dim3(static_cast<unsigned>(items)), dim3(64), 0,
+                         st, g, G.views, G.edges, G.lines, G.pint, prow, pcol, part);
+      hipLaunchKernelGGL((k_bins_reduce<W_FP>), rgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.wfp, part, output, ep);
+    } else {
+      hipLaunchKernelGGL((k_bins_prefix<W_BP>), dim3(static_cast<unsigned>(items)), dim3(64), 0,
+                         st, g, G.views, G.edges, G.lines, G.pint, prow, pcol, part);
+      hipLaunchKernelGGL((k_bins_reduce<W_BP>), rgrid, dim3(kThreads), 0, st, g, G.views,
+                         G.wfp, part, output, ep);
+    }
+  } else if (to_sino(op)) {
     hipLaunchKernelGGL(k_interleave_image<BT>,
                        dim3((g.W + kThreads - 1) / kThreads, g.H, ntile),
                        dim3(kThreads), 0, st, g, input, buf0, buf1);
@@ -687,7 +793,7 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       hipLaunchKernelGGL((k_interleave_sino<BT, false>), sgrid, dim3(kThreads),
                          0, st, g, G.views, G.wfp, input, buf0);
     ProfileScope prof(DD_KERNEL_SINO_TO_IMAGE, st);
-    if (variant == DD_ADJ_QUAD) {
+    if (adj == DD_ADJ_QUAD) {
       dim3 grid((g.W + 2 * kTile - 1) / (2 * kTile), (g.H + 2 * kTile - 1) / (2 * kTile), ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_quad<BT, W_FP>), grid, dim3(kThreads), 0, st, g,
@@ -698,7 +804,7 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       else
         hipLaunchKernelGGL((k_pixels_quad<BT, W_BPW>), grid, dim3(kThreads), 0, st, g,
                            G.views, G.edges, G.lines, G.pint, buf0, output, ep);
-    } else if (variant == DD_ADJ_TILE16) {
+    } else if (adj == DD_ADJ_TILE16) {
       dim3 grid((g.W + 15) / 16, (g.H + 15) / 16, ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 16>), grid, dim3(256), 0, st, g,
@@ -709,7 +815,7 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
       else
         hipLaunchKernelGGL((k_pixels_tile<BT, W_BPW, 16>), grid, dim3(256), 0, st, g,
                            G.views, G.edges, G.lines, G.pint, buf0, output, ep);
-    } else if (variant == DD_ADJ_TILE8) {
+    } else if (adj == DD_ADJ_TILE8) {
       dim3 grid((g.W + 7) / 8, (g.H + 7) / 8, ntile);
       if (op == DD_PROJ_T)
         hipLaunchKernelGGL((k_pixels_tile<BT, W_FP, 8>), grid, dim3(64), 0, st, g,
@@ -826,7 +932,7 @@ size_t dd2d_workspace_size(int scan, int op, const dd_geom2d* g, int batch,
                            int views) {
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return 0;
-  const Layout L = make_layout(q, op, prefix_path(q, op, adjoint_variant(-1, q)));
+  const Layout L = make_layout(q, op, prefix_regions(q, op, adjoint_variant(-1, q)));
   return L.geo_total + L.data_total;
 }
 
@@ -839,7 +945,7 @@ dd_status dd2d_run(int scan, int op, const dd_geom2d* g, const float* input,
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
   const int var = adjoint_variant(-1, q);
-  const Layout L = make_layout(q, op, prefix_path(q, op, var));
+  const Layout L = make_layout(q, op, prefix_regions(q, op, var));
   if (!ws || ws_bytes < L.geo_total + L.data_total) return DD_NO_WORKSPACE;
   auto st = static_cast<hipStream_t>(stream);
   char* w = static_cast<char*>(ws);
@@ -856,7 +962,7 @@ dd_status dd2d_run_noalloc(int scan, int op, const dd_geom2d* g, const float* in
   Geo2 q;
   if (!fill_geo(&q, scan, g, batch, views, pick_tile(batch))) return DD_BAD_ARG;
   const int var = adjoint_variant(-1, q);
-  const Layout L = make_layout(q, op, prefix_path(q, op, var));
+  const Layout L = make_layout(q, op, prefix_regions(q, op, var));
   if (L.geo_total + L.data_total > DD_NOALLOC_SCRATCH_BYTES) return DD_TOO_LARGE;
   char* w = static_cast<char*>(scratch_ptr());
   if (!w) return DD_LAUNCH_FAIL;

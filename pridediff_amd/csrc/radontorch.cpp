@@ -341,7 +341,9 @@ class Plan2d {
         variant_(static_cast<int>(variant)),
         shape_{static_cast<int>(fwd_waves), static_cast<int>(fwd_views),
                static_cast<int>(pix_width)} {
-    TORCH_CHECK((variant >= -1 && variant <= DD_ADJ_QUAD) || variant == DD_ADJ_BLOCK_PREFIX,
+    const int64_t adj = variant < 0 ? variant : (variant & ~static_cast<int64_t>(DD_FWD_PREFIX));
+    TORCH_CHECK(variant == -1 || (variant >= 0 && ((adj >= DD_ADJ_PER_PIXEL && adj <= DD_ADJ_QUAD) ||
+                                                   adj == DD_ADJ_BLOCK_PREFIX)),
                 "Plan2d: unknown kernel variant");
     TORCH_CHECK(fwd_waves == 0 || fwd_waves == 1 || fwd_waves == 2 || fwd_waves == 4,
                 "Plan2d: fwd_waves must be 1, 2 or 4 (0 = default)");

@@ -13,8 +13,12 @@ by tolerance:
   wider than a pixel, and inputs on a 1e3 offset (running sums ~1e3 times
   the values they are differences of); single calls and a batch of 3 at a
   forced tile of 1;
-* dot products against the forward <= 1e-6;
-* BackProjWeighted keeps its walk: equal bit for bit under the variant;
+* with DD_FWD_PREFIX OR-ed in, Proj and BackProjTrans likewise through the
+  lines' running pixel sums (FP32 hi/lo pairs of an FP64 scan), <= 1e-6 of
+  FP64 on the same cases;
+* dot products <= 1e-6 for every pairing of walk and running-sum forward
+  and adjoint;
+* the weighted ops keep their walks: equal bit for bit under the flags;
 * a 512^2 x 720-view slice within 1e-6 of FP64 and of the walk, and the
   default choice there equal bit for bit to the walk;
 * a planned run equals the single call bit for bit, its epilogue
@@ -31,7 +35,8 @@ from pridediff_amd.phantom import ellipses2d
 pytestmark = pytest.mark.gpu
 
 OPNAMES = {"proj": "Proj", "proj_t": "ProjTrans", "bproj": "BackProj",
-           "bproj_w": "BackProjWeighted"}
+           "bproj_t": "BackProjTrans", "bproj_w": "BackProjWeighted",
+           "bproj_tw": "BackProjTransWeighted"}
 GEOMNAMES = {"para": "ParaDis2d", "flat": "FanFlatDis2d", "arc": "FanArcDis2d"}
 
 # (W, H, D, pixel, det scale, pixel shift, bin shift, views)
@@ -102,27 +107,58 @@ def test_single_call_running_sums_match_fp64(rt, scan, offset):
 
 
 @pytest.mark.parametrize("scan", ["para", "flat", "arc"])
+@pytest.mark.parametrize("offset", [0.0, 1000.0])
+def test_single_call_running_sum_forward_matches_fp64(rt, scan, offset):
+    """DD_FWD_PREFIX: Proj and BackProjTrans of single calls through the
+    lines' running pixel sums (hi/lo pairs of an FP64 scan)."""
+    rng = np.random.default_rng(25)
+    try:
+        for batch, tile in ((1, 0), (3, 1)):
+            rt.set_instance_tile(tile)
+            for i, case in enumerate(CASES):
+                g, ang = _geom(scan, case)
+                x = rng.random((batch, g.H, g.W)).astype(np.float32) + np.float32(offset)
+                for op in ("proj", "bproj_t"):
+                    got = _with_variant(rt, rt.ADJ_PER_PIXEL | rt.FWD_PREFIX,
+                                        lambda: _call(rt, scan, op, x, ang, g))
+                    walk = _call(rt, scan, op, x, ang, g)
+                    want = O.op2d(scan, op, x.astype(np.float64), ang, g, "f64")
+                    r, rw = O.rel_l2(got, want), O.rel_l2(walk, want)
+                    print(f"{scan} offset {offset} batch {batch} case {i} {op}: running sums "
+                          f"{r:.2e}, walk {rw:.2e}", flush=True)
+                    assert r <= 1e-6, (scan, offset, batch, i, op, r, rw)
+    finally:
+        rt.set_instance_tile(0)
+
+
+@pytest.mark.parametrize("scan", ["para", "flat", "arc"])
 def test_single_call_running_sums_dot_products(rt, scan):
     rng = np.random.default_rng(22)
+    both = rt.ADJ_BLOCK_PREFIX | rt.FWD_PREFIX
     for i, case in enumerate(CASES):
         g, ang = _geom(scan, case)
         x = rng.random((1, g.H, g.W)).astype(np.float32)
         y = rng.random((1, len(ang), g.D)).astype(np.float32)
-        ax = _call(rt, scan, "proj", x, ang, g).astype(np.float64)
-        aty = _with_variant(rt, rt.ADJ_BLOCK_PREFIX,
-                            lambda: _call(rt, scan, "proj_t", y, ang, g)).astype(np.float64)
-        lhs, rhs = float(np.sum(ax * y)), float(np.sum(x * aty))
-        assert abs(lhs - rhs) / abs(rhs) <= 1e-6, (scan, i, lhs, rhs)
+        for fwd_var, adj_var in ((-1, rt.ADJ_BLOCK_PREFIX), (both, -1), (both, both)):
+            ax = _with_variant(rt, fwd_var,
+                               lambda: _call(rt, scan, "proj", x, ang, g)).astype(np.float64)
+            aty = _with_variant(rt, adj_var,
+                                lambda: _call(rt, scan, "proj_t", y, ang, g)).astype(np.float64)
+            lhs, rhs = float(np.sum(ax * y)), float(np.sum(x * aty))
+            assert abs(lhs - rhs) / abs(rhs) <= 1e-6, (scan, i, fwd_var, adj_var, lhs, rhs)
 
 
 @pytest.mark.parametrize("scan", ["flat", "arc"])
-def test_weighted_backprojection_keeps_its_walk(rt, scan):
+def test_weighted_ops_keep_their_walks(rt, scan):
     rng = np.random.default_rng(23)
     g, ang = _geom(scan, CASES[1])
     y = rng.random((1, len(ang), g.D)).astype(np.float32)
-    a = _with_variant(rt, rt.ADJ_BLOCK_PREFIX, lambda: _call(rt, scan, "bproj_w", y, ang, g))
-    b = _with_variant(rt, rt.ADJ_PER_PIXEL, lambda: _call(rt, scan, "bproj_w", y, ang, g))
-    assert np.array_equal(a, b)
+    x = rng.random((1, g.H, g.W)).astype(np.float32)
+    both = rt.ADJ_BLOCK_PREFIX | rt.FWD_PREFIX
+    for op, inp in (("bproj_w", y), ("bproj_tw", x)):
+        a = _with_variant(rt, both, lambda: _call(rt, scan, op, inp, ang, g))
+        b = _with_variant(rt, rt.ADJ_PER_PIXEL, lambda: _call(rt, scan, op, inp, ang, g))
+        assert np.array_equal(a, b), op
 
 
 def test_full_size_slice_and_default(rt):

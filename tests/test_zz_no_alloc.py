@@ -5,7 +5,7 @@ The no-allocation call takes its scratch from a fixed per-device buffer in
 the library; its results must equal dd2d_run's (caller workspace) bit for
 bit, on two calls in a row where the second is smaller (the buffer keeps the
 first call's contents), for every op of every 2D geometry, and with the
-single-call running-sum adjoint selected.  Problems larger than the buffer
+single-call running sums selected.  Problems larger than the buffer
 are rejected with DD_TOO_LARGE before any device work.
 """
 import ctypes
@@ -98,13 +98,14 @@ def test_noalloc_equals_workspace_call_twice(lib, scan):
 
 @pytest.mark.parametrize("scan", [0, 2])
 def test_noalloc_with_running_sum_adjoint(lib, scan):
-    """With the single-call running-sum adjoint selected (ProjTrans and
-    BackProj at instance tile 1: knots, knot ranges and per-chunk partial
-    images in the scratch too), the same two calls agree bit for bit."""
+    """With the single-call running sums selected (ProjTrans and BackProj at
+    instance tile 1: knots, knot ranges and per-chunk partial images in the
+    scratch too; Proj and BackProjTrans: the lines' running-sum tables), the
+    same two calls agree bit for bit."""
     lib.dd2d_set_adjoint_variant.argtypes = [ctypes.c_int]
-    lib.dd2d_set_adjoint_variant(5)  # DD_ADJ_BLOCK_PREFIX
+    lib.dd2d_set_adjoint_variant(5 | 0x100)  # DD_ADJ_BLOCK_PREFIX | DD_FWD_PREFIX
     try:
-        for op in (1, 2):
+        for op in (0, 1, 2, 3):
             for k, (n, d, views, batch, scale) in enumerate(((512, 736, 720, 1, 1.0),
                                                              (96, 141, 90, 1, 4.0))):
                 g = _geom(scan, n, d, scale)
