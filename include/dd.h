@@ -164,12 +164,14 @@ dd_status dd2d_geometry(int scan, const dd_geom2d* g, const float* angles,
  * walks' 2-3e-7), so the automatic choice keeps the walks and this variant
  * is selected explicitly or by the autotuner. */
 #define DD_ADJ_BLOCK_PREFIX 5
-/* Flag OR-ed into a variant, uneven batches only: the image -> sinogram ops
- * PROJ and BPROJ_T integrate each bin over each line as a difference of the
- * line's running pixel sums (accumulated in FP64, stored as FP32 hi/lo
- * pairs) instead of walking the pixels the bin covers.  Same operator,
- * different rounding (not bit-equal to the walk; a non-finite pixel then
- * reaches every later bin of its line).  BPROJ_TW keeps the walk. */
+/* Flag OR-ed into a variant: the image -> sinogram ops PROJ and BPROJ_T
+ * integrate each bin over each line as a difference of the line's running
+ * pixel sums (accumulated in FP64, stored as FP32 hi/lo pairs) instead of
+ * walking the pixels the bin covers.  In uneven batches, and in dd2d_run /
+ * planned runs at instance tile 1 (dd2d_set_adjoint_variant or a plan's
+ * variant; ignored at larger tiles).  Same operator, different rounding (not
+ * bit-equal to the walk; a non-finite pixel then reaches every later bin of
+ * its line).  BPROJ_TW keeps the walk. */
 #define DD_FWD_PREFIX 0x100
 
 /* `tile` = instances sharing one geometry evaluation (1, 2, 4, 8 or 16;
@@ -277,11 +279,13 @@ void dd_profile_enable(int on);
 int dd_profile_read(int kernel_class, double* total_ms, long long* launches);
 
 /* Default sinogram -> image kernel variant of dd2d_run and of planned runs
- * given variant -1 (DD_ADJ_PER_PIXEL .. DD_ADJ_QUAD or DD_ADJ_BLOCK_PREFIX;
- * -1 = automatic: DD_ADJ_QUAD for up to 4 instances per launch when that
- * still gives 4 workgroups per compute unit, DD_ADJ_PER_PIXEL otherwise).
- * DD_ADJ_BLOCK_PREFIX at a tile above 1 falls back to the automatic choice.
- * dd2d_workspace_size follows the current setting. */
+ * given variant -1 (DD_ADJ_PER_PIXEL .. DD_ADJ_QUAD or DD_ADJ_BLOCK_PREFIX,
+ * optionally | DD_FWD_PREFIX; -1 = automatic: DD_ADJ_QUAD for up to 4
+ * instances per launch when that still gives 4 workgroups per compute unit,
+ * DD_ADJ_PER_PIXEL otherwise, walks in both directions).  The running sums
+ * apply at instance tile 1 only; above it DD_ADJ_BLOCK_PREFIX falls back to
+ * the automatic choice and DD_FWD_PREFIX is ignored.  dd2d_workspace_size
+ * follows the current setting. */
 void dd2d_set_adjoint_variant(int variant);
 
 /* Instance tile used per launch (1, 2, 4, 8 or 16 instances share one
