@@ -94,17 +94,20 @@ PIX_WIDTHS = (16, 32, 64)                                        # per-pixel adj
 
 def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8, 16),
                  variants=None, use_cache: bool = True, shapes: bool = True,
-                 report: dict | None = None) -> BatchedProjector2d:
+                 report: dict | None = None, running_sums: bool = True) -> BatchedProjector2d:
     """Fastest ``BatchedProjector2d`` for this projector, angle set and batch:
-    first the instance tile x adjoint kernel variant (at tile 1 also the
-    running-sum adjoint), then, for the winner,
+    first the instance tile x adjoint kernel variant, then, for the winner,
     the workgroup tile of each gather — forward (63-bin waves x views per
     workgroup; its lines are one of the 8 per-XCD chunks) and per-pixel
-    adjoint (pixels per workgroup row)."""
+    adjoint (pixels per workgroup row).  The walk variants all give the same
+    bits; ``running_sums`` also sweeps, at tile 1, the running-sum adjoint
+    alone and both directions, which round differently (within 1e-6 of the
+    walks)."""
     if variants is None:
         variants = (rt.ADJ_PER_PIXEL, rt.ADJ_TILE16, rt.ADJ_TILE8, rt.ADJ_QUAD)
     opts = [round(float(v), 6) for v in projector.host_options().tolist()]
-    key = f"{_device_key()}|uniform|{projector.scan_type}|{opts}|V={angles.numel()}|B={batch}"
+    key = (f"{_device_key()}|uniform|{projector.scan_type}|{opts}|V={angles.numel()}|B={batch}"
+           f"|rs{int(running_sums)}")
     table = _load() if use_cache else {}
     if key in table:
         if report is not None:
@@ -119,8 +122,10 @@ def tune_uniform(projector, angles: torch.Tensor, batch: int, tiles=(1, 2, 4, 8,
     for t in tiles:
         if t > 1 and t // 2 >= batch:
             continue
-        # the running-sum adjoint runs at tile 1 only
-        for var in tuple(variants) + ((rt.ADJ_BLOCK_PREFIX,) if t == 1 else ()):
+        # the single-call running sums run at tile 1 only
+        extra = ((rt.ADJ_BLOCK_PREFIX, rt.ADJ_BLOCK_PREFIX | rt.FWD_PREFIX)
+                 if t == 1 and running_sums else ())
+        for var in tuple(variants) + extra:
             bp = BatchedProjector2d(projector, angles, batch, tile=t, variant=var)
             with torch.no_grad():
                 results[(t, var)] = time_ms(lambda: (bp.plan.run(rt.PROJ, x),
