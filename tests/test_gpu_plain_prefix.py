@@ -21,9 +21,10 @@ by tolerance:
 * the weighted ops keep their walks: equal bit for bit under the flags;
 * a 512^2 x 720-view slice within 1e-6 of FP64 and of the walk, and the
   default choice there equal bit for bit to the walk;
-* a planned run equals the single call bit for bit, its epilogue
-  (x - A^T y in the kernel's write) equals the composition bit for bit, and
-  a captured graph replays it bit for bit.
+* planned runs (both directions) equal the single calls bit for bit, their
+  epilogues (A x - y in the forward's reduction, x - A^T y in the
+  adjoint's write) equal the compositions bit for bit, and a captured graph
+  of x - A^T (A x - y) replays the eager result bit for bit.
 """
 import numpy as np
 import pytest
@@ -187,20 +188,30 @@ def test_planned_epilogue_and_graph_replay(rt):
                     binshift=0.7, scan_type="arc").cuda()
     v = 97
     ang = torch.from_numpy((np.arange(v) * (2 * np.pi / v) + 0.05).astype(np.float32)).cuda()
-    plan = rt.Plan2d(rt.SCAN[P.scan_type], P.host_options(), ang, 1, 1, rt.ADJ_BLOCK_PREFIX)
+    both = rt.ADJ_BLOCK_PREFIX | rt.FWD_PREFIX
+    plan = rt.Plan2d(rt.SCAN[P.scan_type], P.host_options(), ang, 1, 1, both)
     gen = torch.Generator(device="cuda").manual_seed(5)
     y = torch.rand(1, 1, v, 300, device="cuda", generator=gen)
     x = torch.from_numpy(ellipses2d(120, 1, seed=9)[:, :87, :]).cuda()[:, None].contiguous()
     with torch.no_grad():
         planned = plan.run(rt.PROJ_T, y)
-        single = _with_variant(rt, rt.ADJ_BLOCK_PREFIX, lambda: P.projection_t(y, ang))
+        single = _with_variant(rt, both, lambda: P.projection_t(y, ang))
         assert torch.equal(planned, single)
+        planned_fp = plan.run(rt.PROJ, x)
+        assert torch.equal(planned_fp, _with_variant(rt, both, lambda: P.projection(x, ang)))
         fused = plan.run_ep(rt.PROJ_T, y, c=x, alpha=-1.0, beta=1.0)
         assert torch.equal(fused, x - planned)
-        step = GraphStep(lambda a, b: plan.run_ep(rt.PROJ_T, a, c=b, alpha=-1.0, beta=1.0),
-                         y.clone(), x.clone())
+        # the residual A x - y in the forward's chunk reduction
+        resid = plan.run_ep(rt.PROJ, x, c=y, alpha=1.0, beta=-1.0)
+        assert torch.equal(resid, planned_fp - y)
+
+        def dc_step(a, b):
+            r = plan.run_ep(rt.PROJ, a, c=b, alpha=1.0, beta=-1.0)
+            return plan.run_ep(rt.PROJ_T, r, c=a, alpha=-1.0, beta=1.0)
+        eager = dc_step(x, y)
+        step = GraphStep(dc_step, x.clone(), y.clone())
         for i in range(3):
-            got = step.replay(y, x).clone()
+            got = step.replay(x, y).clone()
             torch.cuda.synchronize()
             _ = torch.randn(100, device="cuda").sum()
-            assert torch.equal(got, fused), f"replay {i}"
+            assert torch.equal(got, eager), f"replay {i}"
