@@ -133,6 +133,11 @@ def roofline_for(key, kernel, ms, b_alg, step_ms=None, n_ops=2):
         # busy fractions of the two halves of the vector-memory path in the
         # counter runs: the addresser (TA) and the data return (TD)
         out["busy_in_counter_run"] = {"ta": k["ta_busy_frac"], "td": k.get("td_busy_frac")}
+        if k.get("lds_busy_frac") is not None:
+            # kernels that read staged data from LDS: its indexed-access
+            # busy share and the part of that stalled by bank conflicts
+            out["busy_in_counter_run"].update(lds=k["lds_busy_frac"],
+                                              lds_bank_conflict=k["lds_bank_conflict_frac"])
     if k:
         out["counts_from"] = ("rocprofv3 --pmc runs of this workload, "
                               "profiles/kernel_counters.md; times measured in this run")
@@ -145,7 +150,11 @@ COUNTER_KEYS = {
     "fanarc": {"fanarc_b16:k_pixels<16,0>": ["k_pixels<16, 0, "],
                "fanarc_b16:k_bins_split<16,0>": ["k_bins_split<16, 0>", "k_bins_reduce<0>"],
                "fanarc_plain:k_pixels<1,0>": ["k_pixels<1, 0, "],
-               "fanarc_plain:k_bins_split<1,0>": ["k_bins_split<1, 0>", "k_bins_reduce<0>"]},
+               "fanarc_plain:k_bins_split<1,0>": ["k_bins_split<1, 0>", "k_bins_reduce<0>"],
+               "fanarc_plain:k_pixels_prefix<0>": ["k_prefix_sino<true>", "k_prefix_ranges",
+                                                   "k_pixels_prefix<0>", "k_sum_chunks"],
+               "fanarc_plain:k_bins_prefix<0>": ["k_prefix_lines", "k_bins_prefix<0>",
+                                                 "k_bins_reduce<0>"]},
     "uneven": {"uneven64:k_batch_quads_window": ["k_batch_quads<0, 8>"],
                "uneven64:k_batch_quads": ["k_batch_quads<0, 0>"],
                "uneven64:k_batch_quads_prefix": ["k_batch_prefix_sino<true>",
@@ -168,7 +177,10 @@ def summarize_counters(base, out_path):
     A (TA/GRBM/SQ), B (TCP/TD), FETCH (FETCH_SIZE) and WRITE (WRITE_SIZE);
     the fanarc FETCH/WRITE passes add ``--copy-probe`` (a 512 MiB device
     copy), whose counters calibrate FETCH_SIZE and WRITE_SIZE to bytes.
-    Values are means over the dispatches of each kernel.
+    An optional pass LDS (SQ_LDS_IDX_ACTIVE, SQ_LDS_BANK_CONFLICT,
+    SQ_INSTS_LDS, GRBM_GUI_ACTIVE) adds the share of cycles the LDS serves
+    indexed reads and writes, and the share of those stalled by bank
+    conflicts.  Values are means over the dispatches of each kernel.
     """
     import collections
     import csv
@@ -206,12 +218,15 @@ def summarize_counters(base, out_path):
     for wl, keys in COUNTER_KEYS.items():
         runs = {p: load(os.path.join(base, f"{wl}_{p}", "run_counter_collection.csv"))
                 for p in ("A", "B", "FETCH", "WRITE")}
+        lds_csv = os.path.join(base, f"{wl}_LDS", "run_counter_collection.csv")
+        Lv = load(lds_csv)[0] if os.path.exists(lds_csv) else None
         for key, pats in keys.items():
             if any(find(runs[p][0], pat) is None for pat in pats for p in runs):
                 continue  # a kernel this run's configuration does not launch
             e = {"kernels": [], "pmc_ms": 0.0, "vmem_instructions": 0.0, "waves": 0.0,
                  "valu_instructions": 0.0, "vmem_bytes": 0.0, "l1_sector_bytes": 0.0,
-                 "fetch_kib_raw": 0.0, "write_kib_raw": 0.0, "_cyc": 0.0, "_ta": 0.0, "_td": 0.0}
+                 "fetch_kib_raw": 0.0, "write_kib_raw": 0.0, "_cyc": 0.0, "_ta": 0.0, "_td": 0.0,
+                 "_lcyc": 0.0, "_lidx": 0.0, "_lbank": 0.0, "_linst": 0.0}
             for pat in pats:
                 (A, tA), (Bv, _), (Fv, _), (Wv, _) = (runs[p] for p in ("A", "B", "FETCH", "WRITE"))
                 ka, kb, kf, kw = find(A, pat), find(Bv, pat), find(Fv, pat), find(Wv, pat)
@@ -227,6 +242,17 @@ def summarize_counters(base, out_path):
                 e["l1_sector_bytes"] += mean(Bv, kb, "TCP_TOTAL_CACHE_ACCESSES_sum") * 32
                 e["fetch_kib_raw"] += mean(Fv, kf, "FETCH_SIZE")
                 e["write_kib_raw"] += mean(Wv, kw, "WRITE_SIZE")
+                kl = find(Lv, pat) if Lv is not None else None
+                if kl is not None:
+                    e["_lcyc"] += mean(Lv, kl, "GRBM_GUI_ACTIVE") / 8
+                    e["_lidx"] += mean(Lv, kl, "SQ_LDS_IDX_ACTIVE")
+                    e["_lbank"] += mean(Lv, kl, "SQ_LDS_BANK_CONFLICT")
+                    e["_linst"] += mean(Lv, kl, "SQ_INSTS_LDS")
+            lcyc, lidx, lbank, linst = (e.pop(k) for k in ("_lcyc", "_lidx", "_lbank", "_linst"))
+            if lcyc > 0 and linst > 0:
+                e["lds_busy_frac"] = lidx / cus / lcyc
+                e["lds_bank_conflict_frac"] = lbank / lidx
+                e["lds_instructions"] = linst
             cyc, ta, td = e.pop("_cyc"), e.pop("_ta"), e.pop("_td")
             e["measured_clock_hz"] = cyc / (e["pmc_ms"] * 1e-3)
             e["ta_busy_frac"], e["td_busy_frac"] = ta / cyc, td / cyc
@@ -240,7 +266,9 @@ def summarize_counters(base, out_path):
         t = v["pmc_ms"] * 1e-3
         print(f"{k:34s} {v['pmc_ms']:8.3f} ms  clk {v['measured_clock_hz'] / 1e9:.2f} GHz  "
               f"TA {v['ta_busy_frac']:.2f}  vmem {v['vmem_bytes'] / t / (cus * 64 * clk):.2f}  "
-              f"HBM {v['hbm_bytes'] / 1e9:7.3f} GB = {v['hbm_bytes'] / t / HBM_PEAK:.3f} of peak")
+              f"HBM {v['hbm_bytes'] / 1e9:7.3f} GB = {v['hbm_bytes'] / t / HBM_PEAK:.3f} of peak"
+              + (f"  LDS {v['lds_busy_frac']:.2f} (bank conflicts {v['lds_bank_conflict_frac']:.2f})"
+                 if "lds_busy_frac" in v else ""))
     print(json.dumps(cal))
 
 
