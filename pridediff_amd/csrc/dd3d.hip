@@ -723,6 +723,10 @@ __global__ __launch_bounds__(kThreads3) void k_voxel_ranges(
 // Floats of LDS the staged k_voxels3d holds one view's rows x columns of
 // running sums in (32 KB).
 constexpr int kStage3Floats = 8192;
+// Staged rows are laid out with an odd number of columns (one unused column
+// after an even count), so that lanes reading the same column on
+// neighbouring rows fall in different LDS banks.
+constexpr bool kStage3OddPitch = true;
 
 // STG: per view, the block of running sums the workgroup's 16 x 16 pixel
 // tile and slice chunk can read is staged in LDS first (rows x columns from
@@ -739,9 +743,9 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
     float* __restrict__ out, const VoxRange* __restrict__ ranges) {
   constexpr int S = slices3<BT>();
   // staged view: rows [srb, srb + sn / sncols) x columns [skb, skb + sncols)
-  // of hi (or the rounded sums) in lq[0 ..), (lo, row value) pairs after them
-  __shared__ float lq[STG ? kStage3Floats : 1];
-  constexpr int kEntry = COMP ? 3 * BT : BT;  // floats per (row, column)
+  // of hi (or the rounded sums) in lq[0 ..), spitch entries per row, and the
+  // (lo, row value) pairs from lq[spair] (16-byte aligned) on
+  __shared__ __align__(16) float lq[STG ? kStage3Floats : 1];
   const int ntx = (g.W + kTile3 - 1) / kTile3;
   const int tile = blockIdx.x;
   const int tiles = ntx * ((g.H + kTile3 - 1) / kTile3);
@@ -771,7 +775,7 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
     // the column magnification m, so the box corners bound it; one row of
     // margin each side (two above: rows r and r + 1 are read).
     bool staged = false;
-    int srb = 0, skb = 0, sncols = 1, sn = 0;
+    int srb = 0, skb = 0, sncols = 1, sn = 0, spitch = 1, spair = 0;
     if constexpr (STG) {
       const VoxRange vr = ranges[static_cast<size_t>(v) * tiles + tile];
       const int l0 = vg.xaxis ? (blockIdx.x / ntx) * kTile3 : (blockIdx.x % ntx) * kTile3;
@@ -795,23 +799,27 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
           static_cast<int>(fmin(fmax(floor(rmax) + 2.0, 0.0), static_cast<double>(g.R + 1)));
       skb = vr.kb;
       sncols = vr.ke - vr.kb;
-      sn = (sre - srb + 1) * sncols;
-      staged = sn * kEntry <= kStage3Floats;
+      spitch = kStage3OddPitch ? (sncols | 1) : sncols;
+      const int nrow = sre - srb + 1;
+      sn = nrow * sncols;
+      spair = (nrow * spitch * BT + 3) & ~3;
+      staged = (COMP ? spair + 2 * nrow * spitch * BT : nrow * spitch * BT) <= kStage3Floats;
       __syncthreads();  // the previous view's walks are done with lq
       if (staged) {
         const size_t vblock = static_cast<size_t>(v) * (g.R + 2) * g.C * g.Bp;
         for (int e = threadIdx.x; e < sn; e += kThreads3) {
           const size_t src = vblock + (static_cast<size_t>(srb + e / sncols) * g.C + skb +
                                        e % sncols) * g.Bp + b0;
+          const int dst = (e / sncols) * spitch + e % sncols;
           float h[BT];
           load_tile<BT>(sino + src, h);
 #pragma unroll
-          for (int b = 0; b < BT; ++b) lq[e * BT + b] = h[b];
+          for (int b = 0; b < BT; ++b) lq[dst * BT + b] = h[b];
           if constexpr (COMP) {
             float lx[2 * BT];
             load_tile<2 * BT>(sino2 + 2 * src, lx);
 #pragma unroll
-            for (int b = 0; b < 2 * BT; ++b) lq[sn * BT + e * 2 * BT + b] = lx[b];
+            for (int b = 0; b < 2 * BT; ++b) lq[spair + dst * 2 * BT + b] = lx[b];
           }
         }
       }
@@ -868,8 +876,8 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
         // the same evaluations as below, reading the staged block: element
         // (row rbase + fu, column k) of the hi block at lb + fu * rsl
         const float wxb = wx * static_cast<float>(z.B);
-        const int lb = ((rbase - srb) * sncols + (k - skb)) * BT;
-        const int rsl = sncols * BT;
+        const int lb = ((rbase - srb) * spitch + (k - skb)) * BT;
+        const int rsl = spitch * BT;
         const __attribute__((address_space(3))) float* lh =
             (const __attribute__((address_space(3))) float*)(lq);
         if constexpr (COMP) {
@@ -881,7 +889,7 @@ __attribute__((amdgpu_waves_per_eu(adj_waves3<BT>()))) void k_voxels3d(
 #pragma unroll
             for (int b = 0; b < BT; ++b) {
               h[b] = lh[e + b];
-              m[b] = fmaf(fr, lh[sn * BT + 2 * (e + b) + 1], lh[sn * BT + 2 * (e + b)]);
+              m[b] = fmaf(fr, lh[spair + 2 * (e + b) + 1], lh[spair + 2 * (e + b)]);
             }
           };
           float hprev[BT], mprev[BT];
