@@ -180,7 +180,12 @@ def summarize_counters(base, out_path):
     An optional pass LDS (SQ_LDS_IDX_ACTIVE, SQ_LDS_BANK_CONFLICT,
     SQ_INSTS_LDS, GRBM_GUI_ACTIVE) adds the share of cycles the LDS serves
     indexed reads and writes, and the share of those stalled by bank
-    conflicts.  Values are means over the dispatches of each kernel.
+    conflicts.  An optional pass VALU (SQ_ACTIVE_INST_VALU,
+    SQ_THREAD_CYCLES_VALU, SQ_INSTS_VALU, SQ_INSTS_VALU_ADD_F64,
+    SQ_INSTS_VALU_FMA_F64, GRBM_GUI_ACTIVE) adds rocprofv3's VALUBusy and
+    VALUUtilization (the share of a wave's lanes active in its VALU
+    instructions) and the FP64 share of the VALU instructions.  Values are
+    means over the dispatches of each kernel.
     """
     import collections
     import csv
@@ -220,13 +225,16 @@ def summarize_counters(base, out_path):
                 for p in ("A", "B", "FETCH", "WRITE")}
         lds_csv = os.path.join(base, f"{wl}_LDS", "run_counter_collection.csv")
         Lv = load(lds_csv)[0] if os.path.exists(lds_csv) else None
+        valu_csv = os.path.join(base, f"{wl}_VALU", "run_counter_collection.csv")
+        Vv = load(valu_csv)[0] if os.path.exists(valu_csv) else None
         for key, pats in keys.items():
             if any(find(runs[p][0], pat) is None for pat in pats for p in runs):
                 continue  # a kernel this run's configuration does not launch
             e = {"kernels": [], "pmc_ms": 0.0, "vmem_instructions": 0.0, "waves": 0.0,
                  "valu_instructions": 0.0, "vmem_bytes": 0.0, "l1_sector_bytes": 0.0,
                  "fetch_kib_raw": 0.0, "write_kib_raw": 0.0, "_cyc": 0.0, "_ta": 0.0, "_td": 0.0,
-                 "_lcyc": 0.0, "_lidx": 0.0, "_lbank": 0.0, "_linst": 0.0}
+                 "_lcyc": 0.0, "_lidx": 0.0, "_lbank": 0.0, "_linst": 0.0,
+                 "_vcyc": 0.0, "_vact": 0.0, "_vthr": 0.0, "_vins": 0.0, "_vf64": 0.0}
             for pat in pats:
                 (A, tA), (Bv, _), (Fv, _), (Wv, _) = (runs[p] for p in ("A", "B", "FETCH", "WRITE"))
                 ka, kb, kf, kw = find(A, pat), find(Bv, pat), find(Fv, pat), find(Wv, pat)
@@ -248,7 +256,21 @@ def summarize_counters(base, out_path):
                     e["_lidx"] += mean(Lv, kl, "SQ_LDS_IDX_ACTIVE")
                     e["_lbank"] += mean(Lv, kl, "SQ_LDS_BANK_CONFLICT")
                     e["_linst"] += mean(Lv, kl, "SQ_INSTS_LDS")
+                kv = find(Vv, pat) if Vv is not None else None
+                if kv is not None:
+                    e["_vcyc"] += mean(Vv, kv, "GRBM_GUI_ACTIVE") / 8
+                    e["_vact"] += mean(Vv, kv, "SQ_ACTIVE_INST_VALU")
+                    e["_vthr"] += mean(Vv, kv, "SQ_THREAD_CYCLES_VALU")
+                    e["_vins"] += mean(Vv, kv, "SQ_INSTS_VALU")
+                    e["_vf64"] += (mean(Vv, kv, "SQ_INSTS_VALU_ADD_F64")
+                                   + mean(Vv, kv, "SQ_INSTS_VALU_FMA_F64"))
             lcyc, lidx, lbank, linst = (e.pop(k) for k in ("_lcyc", "_lidx", "_lbank", "_linst"))
+            vcyc, vact, vthr, vins, vf64 = (e.pop(k) for k in
+                                            ("_vcyc", "_vact", "_vthr", "_vins", "_vf64"))
+            if vcyc > 0 and vact > 0:
+                e["valu_busy"] = vact / cus / vcyc
+                e["valu_lanes_active"] = vthr / (vact * 64)
+                e["valu_f64_share"] = vf64 / vins
             if lcyc > 0 and linst > 0:
                 e["lds_busy_frac"] = lidx / cus / lcyc
                 e["lds_bank_conflict_frac"] = lbank / lidx
