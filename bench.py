@@ -138,6 +138,10 @@ def roofline_for(key, kernel, ms, b_alg, step_ms=None, n_ops=2):
             # busy share and the part of that stalled by bank conflicts
             out["busy_in_counter_run"].update(lds=k["lds_busy_frac"],
                                               lds_bank_conflict=k["lds_bank_conflict_frac"])
+        if k.get("valu_busy") is not None:
+            # rocprofv3's VALUBusy and VALUUtilization (share of lanes active)
+            out["busy_in_counter_run"].update(valu=k["valu_busy"],
+                                              valu_lanes_active=k["valu_lanes_active"])
     if k:
         out["counts_from"] = ("rocprofv3 --pmc runs of this workload, "
                               "profiles/kernel_counters.md; times measured in this run")
