@@ -145,13 +145,40 @@ __global__ __launch_bounds__(kThreads) void k_setup2d(
   setup2d_view(g, angles, views, edges, wfp, lines, pint, blockIdx.x);
 }
 
+// x [B][H][W] -> the row copy and the column copy, by 16 x 16 pixel tiles.
+// The column copy runs down the image's columns, so each tile passes through
+// LDS and is written transposed: a wave's lanes then write contiguous pixels
+// of both copies (row by row, a lane's column-copy pixel lay H pixels from
+// its neighbour's).
+constexpr int kItile = 16;
+
 template <int BT>
-__global__ __launch_bounds__(kThreads) void k_interleave_image(
+__global__ __launch_bounds__(kItile * kItile) void k_interleave_image(
     Geo2 g, const float* __restrict__ x, float* __restrict__ xrow,
     float* __restrict__ xcol) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= g.W) return;
-  interleave_image_px<BT>(g, x, xrow, xcol, blockIdx.y, w, blockIdx.z * BT);
+  // pixel (r, c) of the tile, value b, at t[r * kRowPitch + c * (BT + 1) + b]:
+  // odd strides, so neither the row-wise writes nor the column-wise reads
+  // below pile onto one LDS bank
+  constexpr int kRowPitch = kItile * (BT + 1) + 1;
+  __shared__ float t[kItile * kRowPitch];
+  const int tx = static_cast<int>(threadIdx.x) % kItile, ty = static_cast<int>(threadIdx.x) / kItile;
+  const int h0 = blockIdx.y * kItile, w0 = blockIdx.x * kItile, b0 = blockIdx.z * BT;
+  if (h0 + ty < g.H && w0 + tx < g.W) {
+    float v[BT];
+    load_image_px<BT>(g, x, h0 + ty, w0 + tx, b0, v);
+    store_image_px<BT, false>(g, xrow, h0 + ty, w0 + tx, b0, v);
+#pragma unroll
+    for (int b = 0; b < BT; ++b) t[ty * kRowPitch + tx * (BT + 1) + b] = v[b];
+  }
+  __syncthreads();
+  // transposed: consecutive lanes take consecutive rows of one image
+  // column, which are consecutive pixels of the column copy
+  if (h0 + tx < g.H && w0 + ty < g.W) {
+    float v[BT];
+#pragma unroll
+    for (int b = 0; b < BT; ++b) v[b] = t[tx * kRowPitch + ty * (BT + 1) + b];
+    store_image_px<BT, true>(g, xcol, h0 + tx, w0 + ty, b0, v);
+  }
 }
 
 template <int BT, bool FOLD_W>
@@ -731,8 +758,8 @@ dd_status launch_tile(const Geo2& g, int op, int variant, const float* input,
     }
   } else if (to_sino(op)) {
     hipLaunchKernelGGL(k_interleave_image<BT>,
-                       dim3((g.W + kThreads - 1) / kThreads, g.H, ntile),
-                       dim3(kThreads), 0, st, g, input, buf0, buf1);
+                       dim3((g.W + kItile - 1) / kItile, (g.H + kItile - 1) / kItile, ntile),
+                       dim3(kItile * kItile), 0, st, g, input, buf0, buf1);
     auto* part = reinterpret_cast<float*>(ws + L.part);
     const int kb_n = (g.D + kThreads - 1) / kThreads;
     const int waves = sh.fwd_waves, vpb = sh.fwd_views;

@@ -203,30 +203,41 @@ __device__ __forceinline__ void load_sino(const Geo2& g, const float* __restrict
 }
 
 template <int BT>
-__device__ __forceinline__ void interleave_image_px(
-    const Geo2& g, const float* __restrict__ x, float* __restrict__ xrow,
-    float* __restrict__ xcol, int h, int w, int b0) {
-  float v[BT];
+__device__ __forceinline__ void load_image_px(const Geo2& g, const float* __restrict__ x, int h,
+                                              int w, int b0, float (&v)[BT]) {
 #pragma unroll
   for (int b = 0; b < BT; ++b)
     v[b] = (b0 + b < g.B)
                ? x[(static_cast<size_t>(b0 + b) * g.H + h) * g.W + w]
                : 0.0f;
+}
+
+// Pixel (h, w)'s BT values into the row copy (COL false) or the column copy
+// (COL true, lines read bottom-up).
+template <int BT, bool COL>
+__device__ __forceinline__ void store_image_px(const Geo2& g, float* __restrict__ xc, int h,
+                                               int w, int b0, const float (&v)[BT]) {
+  const size_t px = COL ? static_cast<size_t>(w) * g.H + (g.H - 1 - h)
+                        : static_cast<size_t>(h) * g.W + w;
   if constexpr (image_quad_major<BT>()) {
     const size_t plane = static_cast<size_t>(g.W) * g.H;  // pixels per quad plane
-    const size_t pr = static_cast<size_t>(h) * g.W + w;
-    const size_t pc = static_cast<size_t>(w) * g.H + (g.H - 1 - h);
 #pragma unroll
-    for (int q = 0; q < BT / 4; ++q) {
-      const float4 t = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
-      const size_t pq = static_cast<size_t>(b0 / 4 + q) * plane;
-      *reinterpret_cast<float4*>(xrow + (pq + pr) * 4) = t;
-      *reinterpret_cast<float4*>(xcol + (pq + pc) * 4) = t;
-    }
+    for (int q = 0; q < BT / 4; ++q)
+      *reinterpret_cast<float4*>(xc + (static_cast<size_t>(b0 / 4 + q) * plane + px) * 4) =
+          make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
   } else {
-    store_tile<BT>(xrow + (static_cast<size_t>(h) * g.W + w) * g.Bp + b0, v);
-    store_tile<BT>(xcol + (static_cast<size_t>(w) * g.H + (g.H - 1 - h)) * g.Bp + b0, v);
+    store_tile<BT>(xc + px * g.Bp + b0, v);
   }
+}
+
+template <int BT>
+__device__ __forceinline__ void interleave_image_px(
+    const Geo2& g, const float* __restrict__ x, float* __restrict__ xrow,
+    float* __restrict__ xcol, int h, int w, int b0) {
+  float v[BT];
+  load_image_px<BT>(g, x, h, w, b0, v);
+  store_image_px<BT, false>(g, xrow, h, w, b0, v);
+  store_image_px<BT, true>(g, xcol, h, w, b0, v);
 }
 
 // p [B][V][D] -> s [V][D][Bp] in walk order, optionally scaled by the FP

@@ -306,27 +306,49 @@ __global__ __launch_bounds__(kThreads3) void k_setup3d(
 // ---------------------------------------------------------------------------
 // x [B][Dz][H][W] -> xrow [tile][H][Dz][W][BT], xcol [tile][W][Dz][H][BT]
 // (bottom-up), tile = BT consecutive instances: a pixel's BT values and its
-// neighbour's are contiguous, whatever the padded batch.
+// neighbour's are contiguous, whatever the padded batch.  One workgroup per
+// 16 x 16 pixels of one slice: the column copy runs down the image's
+// columns, so the tile passes through LDS and is written transposed, and a
+// wave's lanes write contiguous pixels of both copies.
+constexpr int kItile3 = 16;
+
 template <int BT>
-__global__ __launch_bounds__(kThreads3) void k_interleave_volume(
+__global__ __launch_bounds__(kItile3 * kItile3) void k_interleave_volume(
     Geo3 g, const float* __restrict__ x, float* __restrict__ xrow,
     float* __restrict__ xcol) {
-  const int nwc = (g.W + kThreads3 - 1) / kThreads3;
-  const int w = (blockIdx.x % nwc) * kThreads3 + threadIdx.x;
-  const int h = blockIdx.x / nwc;
+  // pixel (r, c) of the tile, value b, at t[r * kRowPitch + c * (BT + 1) + b]
+  // (odd strides: neither access order below piles onto one LDS bank)
+  constexpr int kRowPitch = kItile3 * (BT + 1) + 1;
+  __shared__ float t[kItile3 * kRowPitch];
+  const int ntw = (g.W + kItile3 - 1) / kItile3;
+  const int tx = static_cast<int>(threadIdx.x) % kItile3;
+  const int ty = static_cast<int>(threadIdx.x) / kItile3;
+  const int w0 = (blockIdx.x % ntw) * kItile3, h0 = (blockIdx.x / ntw) * kItile3;
   const int s = blockIdx.y;
   const int b0 = blockIdx.z * BT;
-  if (w >= g.W) return;
-  float v[BT];
-#pragma unroll
-  for (int b = 0; b < BT; ++b)
-    v[b] = (b0 + b < g.B)
-               ? x[((static_cast<size_t>(b0 + b) * g.Dz + s) * g.H + h) * g.W + w]
-               : 0.0f;
   const size_t tile = static_cast<size_t>(blockIdx.z) * g.W * g.H * g.Dz * BT;
-  store_tile<BT>(xrow + tile + ((static_cast<size_t>(h) * g.Dz + s) * g.W + w) * BT, v);
-  store_tile<BT>(xcol + tile + ((static_cast<size_t>(w) * g.Dz + s) * g.H + (g.H - 1 - h)) * BT,
-                 v);
+  if (h0 + ty < g.H && w0 + tx < g.W) {
+    const int h = h0 + ty, w = w0 + tx;
+    float v[BT];
+#pragma unroll
+    for (int b = 0; b < BT; ++b)
+      v[b] = (b0 + b < g.B)
+                 ? x[((static_cast<size_t>(b0 + b) * g.Dz + s) * g.H + h) * g.W + w]
+                 : 0.0f;
+    store_tile<BT>(xrow + tile + ((static_cast<size_t>(h) * g.Dz + s) * g.W + w) * BT, v);
+#pragma unroll
+    for (int b = 0; b < BT; ++b) t[ty * kRowPitch + tx * (BT + 1) + b] = v[b];
+  }
+  __syncthreads();
+  // transposed: consecutive lanes take consecutive rows of one image column
+  if (h0 + tx < g.H && w0 + ty < g.W) {
+    const int h = h0 + tx, w = w0 + ty;
+    float v[BT];
+#pragma unroll
+    for (int b = 0; b < BT; ++b) v[b] = t[tx * kRowPitch + ty * (BT + 1) + b];
+    store_tile<BT>(xcol + tile + ((static_cast<size_t>(w) * g.Dz + s) * g.H + (g.H - 1 - h)) * BT,
+                   v);
+  }
 }
 
 // p [B][V][R][C] -> Q [V][R+2][C][Bp]: per view and column, the running sum
@@ -1183,8 +1205,9 @@ dd_status launch3(const Geo3& g, int op, const float* input, float* output,
   const int ntile = g.Bp / BT;
   if (to_sino3(op)) {
     hipLaunchKernelGGL(k_interleave_volume<BT>,
-                       dim3((g.W + kThreads3 - 1) / kThreads3 * g.H, g.Dz, ntile),
-                       dim3(kThreads3), 0, st, g, input, buf0, buf1);
+                       dim3(((g.W + kItile3 - 1) / kItile3) * ((g.H + kItile3 - 1) / kItile3),
+                            g.Dz, ntile),
+                       dim3(kItile3 * kItile3), 0, st, g, input, buf0, buf1);
     constexpr int S = rows3<BT>();
     const int nch = (g.R + S - 1) / S;
     const unsigned ncolb = (g.C + kColsPerBlock - 1) / kColsPerBlock;
