@@ -136,6 +136,29 @@ def test_batch_tiles_are_consistent(rt):
         assert np.array_equal(whole, single), op
 
 
+@pytest.mark.parametrize("geom", ["para", "flat", "arc"])
+def test_batch_tiles_on_partial_pixel_tiles(rt, geom):
+    """A batch of 16 (quad-major copies, built through 16 x 16-pixel LDS
+    tiles) and a batch of 3 (tile 4) equal single-instance calls bit for bit
+    on a rectangular image whose sides are not multiples of 16."""
+    if geom == "para":
+        g = O.Geom2D(70, 53, 97, 1.1, 1.1, 1.3, 1.0, 1.0, 0.4, -0.6, 0.2)
+    elif geom == "flat":
+        g = O.Geom2D(70, 53, 97, 1.1, 1.1, 1.3, 595.0, 1085.6, 0.4, -0.6, 0.2)
+    else:
+        g = O.Geom2D(70, 53, 97, 1.1, 1.1, 1.3 / 1085.6, 595.0, 1085.6, 0.4, -0.6, 0.2 / 1085.6)
+    ang = _angles(40)
+    rng = np.random.default_rng(31)
+    ops = ("proj", "bproj_t") if geom == "para" else ("proj", "bproj_t", "bproj_tw")
+    for batch in (16, 3):
+        x = rng.random((batch, 53, 70)).astype(np.float32)
+        for op in ops:
+            whole = gpu_op(rt, geom, op, x, ang, g)
+            single = np.concatenate([gpu_op(rt, geom, op, x[i:i + 1], ang, g)
+                                     for i in range(batch)])
+            assert np.array_equal(whole, single), (geom, batch, op)
+
+
 def test_autograd_gradient_and_finite_differences(rt):
     from pridediff_amd.deeprecon.torch.projector import projector2d
     n, d, v = 64, 92, 60
