@@ -148,8 +148,8 @@ __global__ __launch_bounds__(kThreads) void k_setup2d(
 // x [B][H][W] -> the row copy and the column copy, by 16 x 16 pixel tiles.
 // The column copy runs down the image's columns, so each tile passes through
 // LDS and is written transposed: a wave's lanes then write contiguous pixels
-// of both copies (row by row, a lane's column-copy pixel lay H pixels from
-// its neighbour's).
+// of both copies (written row by row, neighbouring lanes' column-copy pixels
+// lie H pixels apart).
 constexpr int kItile = 16;
 
 template <int BT>
@@ -161,7 +161,8 @@ __global__ __launch_bounds__(kItile * kItile) void k_interleave_image(
   // below pile onto one LDS bank
   constexpr int kRowPitch = kItile * (BT + 1) + 1;
   __shared__ float t[kItile * kRowPitch];
-  const int tx = static_cast<int>(threadIdx.x) % kItile, ty = static_cast<int>(threadIdx.x) / kItile;
+  const int tx = static_cast<int>(threadIdx.x) % kItile;
+  const int ty = static_cast<int>(threadIdx.x) / kItile;
   const int h0 = blockIdx.y * kItile, w0 = blockIdx.x * kItile, b0 = blockIdx.z * BT;
   if (h0 + ty < g.H && w0 + tx < g.W) {
     float v[BT];
