@@ -76,8 +76,8 @@ dd_status dd3d_run(int scan, int op, const dd_geom3d* g, const float* input,
 void dd3d_set_compensated(int mode);
 /* Whether the sinogram -> volume gather stages each view's row running sums
  * for its 16 x 16 pixel tile in LDS: 0 never, 1 always, 2 (default)
- * automatic — with compensated sums or batches of 1-2 (C3-like cone-flat:
- * adjoint 20.0 -> 17.1 ms), not for rounded sums at batches of 3+ (helical
+ * automatic — with compensated sums or batches of 1-2 (cone-flat 256^3 x 2:
+ * adjoint 20.0 -> 15.6 ms), not for rounded sums at batches of 3+ (helical
  * 512x512x64 x 4: 94 ms unstaged, 105 ms staged).  The same bits either
  * way; a view whose block exceeds 32 KB of LDS is read from global memory.
  * No counterpart in the reference. */
@@ -85,9 +85,9 @@ void dd3d_set_adjoint_staging(int mode);
 /* Dispatch order of the volume -> sinogram gather: 0 views first (two
  * detector-row chunks per workgroup), 1 detector columns first (one chunk),
  * 2 (default) automatic: columns first when one instance tile's volume copy
- * exceeds 192 MB (C4-like helical 512x512x64 x 4, 268 MB: forward 75.4 ->
- * 65.7 ms), views first otherwise (C3-like cone-flat 256^3 x 2, 134 MB: 17.7
- * ms views first, 19.5 ms columns first).  The same bits either way.  No
+ * exceeds 192 MB (helical 512x512x64 x 4, 268 MB: forward 75.4 -> 65.7 ms),
+ * views first otherwise (cone-flat 256^3 x 2, 134 MB: 17.7 ms views first,
+ * 19.5 ms columns first).  The same bits either way.  No
  * counterpart in the reference. */
 void dd3d_set_forward_order(int mode);
 

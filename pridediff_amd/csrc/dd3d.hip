@@ -42,13 +42,15 @@ constexpr int kThreads3 = 256;
 //    tiles), two chunks per workgroup.  Consecutive workgroups are
 //    consecutive views of one block, so the blocks in flight sweep the whole
 //    volume copy; best while that copy stays in the 256 MB Infinity Cache
-//    (C3 17.5 ms; with one / four chunks per workgroup 17.3 / 17.7 ms, C4
-//    77.4 / 77.6 against 75.4 at two).
+//    (cone-flat 256^3 x 2: 17.5 ms; with one / four chunks per workgroup
+//    17.3 / 17.7 ms; helical 512x512x64 x 4: 77.4 / 77.6 against 75.4 at
+//    two).
 //  * columns first: grid (column blocks, views, row chunks x instance tiles),
 //    one chunk per workgroup.  The workgroups in flight are the column blocks
 //    of a few hundred consecutive views for one row chunk: a band of slices,
 //    and on each XCD a few column wedges of it.  Best once the copy outgrows
-//    the cache (C4, 268 MB: 75.4 -> 65.7 ms; C3 19.5 ms).
+//    the cache (the helical copy, 268 MB: 75.4 -> 65.7 ms; cone-flat
+//    19.5 ms).
 constexpr int kColsPerBlock = 64;
 constexpr int kFwdThreads = 2 * kColsPerBlock;
 constexpr int kChunksViewsFirst = 2;

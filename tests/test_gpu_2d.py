@@ -87,11 +87,11 @@ def test_fan_arc_c1_projection_parity(rt):
     x = ellipses2d(512, 2, seed=3, scale=0.04)
     got = gpu_op(rt, "arc", "proj", x, ang, g)
     f64 = O.op2d("arc", "proj", x[:1], ang, g, "f64")
-    assert report("C1 arc proj vs f64", got[:1], f64) <= 1e-5
+    assert report("fan-arc 512^2 proj vs f64", got[:1], f64) <= 1e-5
     if O.ref_available():
         ref = O.op2d("arc", "proj", x[:1], ang, g, "ref")
         floor = O.rel_l2(ref, f64)
-        assert report(f"C1 arc proj vs reference (floor {floor:.2e})", got[:1], ref) <= max(1e-5, 2 * floor)
+        assert report(f"fan-arc 512^2 proj vs reference (floor {floor:.2e})", got[:1], ref) <= max(1e-5, 2 * floor)
 
 
 def test_fan_arc_c1_adjoint_parity(rt):
@@ -99,11 +99,11 @@ def test_fan_arc_c1_adjoint_parity(rt):
     y = np.random.default_rng(4).uniform(0, 1, (1, 720, 736)).astype(np.float32)
     got = gpu_op(rt, "arc", "proj_t", y, ang, g)
     f64 = O.op2d("arc", "proj_t", y, ang, g, "f64")
-    assert report("C1 arc proj_t vs f64", got, f64) <= 1e-5
+    assert report("fan-arc 512^2 proj_t vs f64", got, f64) <= 1e-5
     if O.ref_available():
         ref = O.op2d("arc", "proj_t", y, ang, g, "ref")
         floor = O.rel_l2(ref, f64)
-        assert report(f"C1 arc proj_t vs reference (floor {floor:.2e})", got, ref) <= max(1e-5, 2 * floor)
+        assert report(f"fan-arc 512^2 proj_t vs reference (floor {floor:.2e})", got, ref) <= max(1e-5, 2 * floor)
 
 
 @pytest.mark.parametrize("geom", ["para", "flat", "arc"])
@@ -248,9 +248,9 @@ def test_parallel_c0_workload(rt):
     for op, inp in (("proj", x), ("bproj", sino), ("proj_t", sino)):
         got = gpu_op(rt, "para", op, inp, ang, g)
         want = O.op2d("para", op, inp, ang, g, "f64")
-        assert report(f"C0 para {op} vs FP64", got, want) <= 1e-5
+        assert report(f"para 256^2 {op} vs FP64", got, want) <= 1e-5
         if O.ref_available():
             ref = O.op2d("para", op, inp, ang, g, "ref")
             floor = O.rel_l2(ref, want)
-            r = report(f"C0 para {op} vs reference (its floor {floor:.2e})", got, ref)
+            r = report(f"para 256^2 {op} vs reference (its floor {floor:.2e})", got, ref)
             assert r <= max(1e-5, 2 * floor)

@@ -50,7 +50,7 @@ def sparse_views(y, sel):
     return out
 
 
-# ---------------------------------------------------------------- fan-arc C1
+# ------------------------------------------------- fan-arc 512^2, batch 16
 def test_fanarc_batch16_planned_full_size():
     from pridediff_amd.batched import BatchedProjector2d
     from pridediff_amd.deeprecon.torch.projector import projector2d
@@ -67,22 +67,22 @@ def test_fanarc_batch16_planned_full_size():
         s = BP.projection(x)
     pick = [0, 15]
     want = O.op2d("arc", "proj", x_np[pick], ang_np, g, "f64")
-    assert report("C1 batch-16 forward (slices 0, 15) vs FP64", s[pick, 0].cpu().numpy(), want) <= 1e-5
+    assert report("fan-arc 512^2 batch-16 forward (slices 0, 15) vs FP64", s[pick, 0].cpu().numpy(), want) <= 1e-5
     gen = torch.Generator(device="cuda").manual_seed(5)
     y = torch.rand(b, 1, v, d, device="cuda", generator=gen)
     with torch.no_grad():
         aty = BP.projection_t(y)
     want_t = O.op2d("arc", "proj_t", y[pick, 0].cpu().numpy(), ang_np, g, "f64")
-    assert report("C1 batch-16 adjoint (slices 0, 15) vs FP64", aty[pick, 0].cpu().numpy(),
+    assert report("fan-arc 512^2 batch-16 adjoint (slices 0, 15) vs FP64", aty[pick, 0].cpu().numpy(),
                   want_t) <= 1e-5
     xr = torch.rand(b, 1, n, n, device="cuda", generator=gen)
     with torch.no_grad():
         rel, lhs, rhs = dot_rel(BP.projection(xr), y, xr, aty)
-    print(f"C1 dot-product <Ax,y>={lhs:.10e} <x,A^T y>={rhs:.10e} rel={rel:.2e}", flush=True)
+    print(f"fan-arc 512^2 dot-product <Ax,y>={lhs:.10e} <x,A^T y>={rhs:.10e} rel={rel:.2e}", flush=True)
     assert rel <= 1e-6
 
 
-# ------------------------------------------------------- uneven fan-arc C2
+# ------------------------------------------- 64 uneven fan-arc instances
 def _c2_instances():
     """The 64-instance set of the benchmark (same seeds)."""
     from pridediff_amd.deeprecon.torch.projector import projector2d
@@ -124,7 +124,7 @@ def test_uneven_64_forward_every_instance_vs_fp64(c2):
         r = O.rel_l2(s[sel].cpu().numpy()[None], want)
         worst = max(worst, r)
         assert r <= 1e-5, (i, (h, w, v, d), r)
-    print(f"C2 forward, 64 instances x 4 views vs FP64: worst rel_l2 {worst:.3e}", flush=True)
+    print(f"uneven batch forward, 64 instances x 4 views vs FP64: worst rel_l2 {worst:.3e}", flush=True)
 
 
 def test_uneven_64_adjoint_every_instance_vs_fp64(c2):
@@ -146,7 +146,7 @@ def test_uneven_64_adjoint_every_instance_vs_fp64(c2):
         r = O.rel_l2(b.cpu().numpy()[None], want)
         worst = max(worst, r)
         assert r <= 1e-5, (i, (h, w, v, d), r)
-    print(f"C2 adjoint, 64 instances x 4 views vs FP64: worst rel_l2 {worst:.3e}", flush=True)
+    print(f"uneven batch adjoint, 64 instances x 4 views vs FP64: worst rel_l2 {worst:.3e}", flush=True)
 
 
 def test_uneven_64_dot_product_per_instance(c2):
@@ -164,7 +164,7 @@ def test_uneven_64_dot_product_per_instance(c2):
         rel, lhs, rhs = dot_rel([a], [yy], [xx], [b])
         worst = max(worst, rel)
         assert rel <= 1e-6, (i, ub.shapes[i], lhs, rhs)
-    print(f"C2 dot-product, 64 instances: worst rel {worst:.2e}", flush=True)
+    print(f"uneven batch dot-product, 64 instances: worst rel {worst:.2e}", flush=True)
 
 
 def test_uneven_64_running_sums_vs_fp64_and_dot(c2):
@@ -210,7 +210,7 @@ def test_uneven_64_running_sums_vs_fp64_and_dot(c2):
         rel, lhs, rhs = dot_rel([a], [yy], [xx], [b])
         worst_dot = max(worst_dot, rel)
         assert rel <= 1e-6, (i, ub.shapes[i], lhs, rhs)
-    print(f"C2 running sums, 64 instances x 4 views vs FP64: forward worst rel_l2 {worst_f:.3e}, "
+    print(f"uneven batch running sums, 64 instances x 4 views vs FP64: forward worst rel_l2 {worst_f:.3e}, "
           f"adjoint {worst_a:.3e}; dot-product worst rel {worst_dot:.2e}", flush=True)
 
 
@@ -247,7 +247,7 @@ def test_uneven_64_small_grid_strides_items(c2):
             ub.plan.set_variant(rt.ADJ_QUAD)
 
 
-# --------------------------------------------------------- cone-flat C3
+# ----------------------------------------------- cone-flat 256^3, batch 2
 @pytest.fixture(scope="module")
 def c3():
     from pridediff_amd.deeprecon.torch.projector import projector3d
@@ -271,7 +271,7 @@ def test_cone_flat_full_size_forward_vs_fp64(c3):
         s = P.projection(x, ang)
     got = s[:, 0, C3_SEL].cpu().numpy()
     want = O.op3d("cone_flat", "proj", x_np, ang_np[C3_SEL], g)
-    assert report("C3 cone-flat forward, 5 views x 2 volumes vs FP64", got, want) <= 1e-5
+    assert report("cone-flat 256^3 forward, 5 views x 2 volumes vs FP64", got, want) <= 1e-5
 
 
 def test_cone_flat_full_size_adjoint_vs_fp64(c3):
@@ -283,7 +283,7 @@ def test_cone_flat_full_size_adjoint_vs_fp64(c3):
     x = torch.zeros(2, 1, g.Dz, g.H, g.W, device="cuda", requires_grad=True)
     P.projection(x, ang).backward(y)          # the autograd backward is ProjTrans
     want = O.op3d("cone_flat", "proj_t", y[:, 0, C3_SEL].cpu().numpy(), ang_np[C3_SEL], g)
-    assert report("C3 cone-flat adjoint (autograd), 5 views vs FP64",
+    assert report("cone-flat 256^3 adjoint (autograd), 5 views vs FP64",
                   x.grad[:, 0].cpu().numpy(), want) <= 1e-5
 
 
@@ -295,11 +295,11 @@ def test_cone_flat_full_size_dot_product(c3):
     y = torch.rand(2, 1, len(ang_np), g.R, g.C, device="cuda", generator=gen)
     with torch.no_grad():
         rel, lhs, rhs = dot_rel(P.projection(x, ang), y, x, P.projection_t(y, ang))
-    print(f"C3 dot-product <Ax,y>={lhs:.10e} <x,A^T y>={rhs:.10e} rel={rel:.2e}", flush=True)
+    print(f"cone-flat 256^3 dot-product <Ax,y>={lhs:.10e} <x,A^T y>={rhs:.10e} rel={rel:.2e}", flush=True)
     assert rel <= 1e-6
 
 
-# ----------------------------------------------------------- helical C4
+# -------------------------------------------- helical 512x512x64, batch 4
 @pytest.fixture(scope="module")
 def c4():
     from pridediff_amd.deeprecon.torch.projector import projectorsp
@@ -330,7 +330,7 @@ def test_helical_full_size_forward_vs_fp64(c4):
     del s
     want = O.op3d("spiral", "proj", x_np[pick], ang_np[C4_SEL], g,
                   np.ascontiguousarray(spos[:, C4_SEL]))
-    assert report("C4 helical forward, 5 views x 2 volumes vs FP64", got, want) <= 1e-5
+    assert report("helical 512x512x64 forward, 5 views x 2 volumes vs FP64", got, want) <= 1e-5
 
 
 def test_helical_full_size_adjoint_vs_fp64(c4):
@@ -345,7 +345,7 @@ def test_helical_full_size_adjoint_vs_fp64(c4):
     pick = [1, 2]
     want = O.op3d("spiral", "proj_t", y[pick, 0][:, sel].cpu().numpy(), ang_np[C4_SEL], g,
                   np.ascontiguousarray(spos[:, C4_SEL]))
-    assert report("C4 helical adjoint (autograd), 5 views vs FP64",
+    assert report("helical 512x512x64 adjoint (autograd), 5 views vs FP64",
                   x.grad[pick, 0].cpu().numpy(), want) <= 1e-5
 
 
@@ -359,5 +359,5 @@ def test_helical_full_size_dot_product(c4):
         ax = P.projection(x, ang, sp)
         aty = P.projection_t(y, ang, sp)
         rel, lhs, rhs = dot_rel(ax, y, x, aty)
-    print(f"C4 dot-product <Ax,y>={lhs:.10e} <x,A^T y>={rhs:.10e} rel={rel:.2e}", flush=True)
+    print(f"helical 512x512x64 dot-product <Ax,y>={lhs:.10e} <x,A^T y>={rhs:.10e} rel={rel:.2e}", flush=True)
     assert rel <= 1e-6
