@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kThreads) void k_prefix_ranges(
 // view whose range exceeds the buffer is walked from global memory.
 template <int WM>
 __global__ __launch_bounds__(kPrefThreads)
-__attribute__((amdgpu_waves_per_eu(4))) void k_pixels_prefix(
+__attribute__((amdgpu_waves_per_eu(5))) void k_pixels_prefix(
     Geo2 g, const ViewGeo* __restrict__ views, const LineGeo* __restrict__ lines,
     const float* __restrict__ pint, const double2* __restrict__ psino,
     const int2* __restrict__ kr, float* __restrict__ part, int vchunk) {

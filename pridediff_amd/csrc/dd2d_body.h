@@ -1124,7 +1124,7 @@ __device__ __forceinline__ void prefix_sino_view(
 // kn[k - kmin]; the walk stays within bins kmin .. kmax - 1, which must hold
 // every bin the run's pixel edges fall in (then the result does not depend
 // on the range).  KP is a global or an LDS pointer to the knots as pairs of
-// doubles {e, R}.  Positions are unclamped FP64 pixel units.
+// doubles {e, R}.
 template <typename KP>
 __device__ __forceinline__ double2 knot_at(KP kn, int i) {
   return make_double2(kn[2 * i], kn[2 * i + 1]);
@@ -1134,42 +1134,50 @@ template <int NP, typename KP>
 __device__ __forceinline__ void pixel_run_view_prefix(
     const Geo2& g, const ViewGeo& vg, const LineGeo& lg, float pl, float gw, KP kn, int kmin,
     int kmax, int ib, int pf, int plst, float (&acc)[NP]) {
-  const int ifirst = ib + pf;
-  const double b = lg.a0 * lg.inv_p;  // t = e / P - a0 / P
-  auto pos = [&](double e) { return fma(e, lg.inv_p, -b); };
+  // Positions relative to the run's edge ib (tau = e / P - a0 / P - ib) and
+  // running sums relative to the first knot read (rho = R - rbase) are
+  // formed in FP64 once per knot and kept in FP32: over a run of NP pixels
+  // both stay of the size of a few pixels and bins, so FP32 holds them to
+  // its precision relative to the values the pixels receive.
+  const double bb = fma(lg.a0, lg.inv_p, static_cast<double>(ib));
+  auto tau = [&](double e) { return static_cast<float>(fma(e, lg.inv_p, -bb)); };
   // the bin whose lower knot lies at or before the run's first pixel edge
   // (guessed at that edge)
-  int k = min(max(guess_bin_fast(g, vg, lg.a0 + ifirst * static_cast<double>(pl)), kmin),
+  int k = min(max(guess_bin_fast(g, vg, lg.a0 + (ib + pf) * static_cast<double>(pl)), kmin),
               kmax - 1);
-  double2 k1 = knot_at(kn, k - kmin);
-  double t1 = pos(k1.x);
-  while (k > kmin && t1 > ifirst) {
+  double2 q = knot_at(kn, k - kmin);
+  const double rbase = q.y;
+  float t1 = tau(q.x), r1 = 0.0f;
+  const float fpf = static_cast<float>(pf);
+  while (k > kmin && t1 > fpf) {
     --k;
-    k1 = knot_at(kn, k - kmin);
-    t1 = pos(k1.x);
+    q = knot_at(kn, k - kmin);
+    t1 = tau(q.x);
+    r1 = static_cast<float>(q.y - rbase);
   }
-  double2 k2 = knot_at(kn, k + 1 - kmin);
-  double t2 = pos(k2.x);
-  float rw = __builtin_amdgcn_rcpf(static_cast<float>(t2 - t1));
+  q = knot_at(kn, k + 1 - kmin);
+  float t2 = tau(q.x), r2 = static_cast<float>(q.y - rbase);
+  float rw = __builtin_amdgcn_rcpf(t2 - t1);
   const float scale = static_cast<float>(lg.inv_p) * gw;
-  double gprev = 0.0;
+  float gprev = 0.0f;
 #pragma unroll
   for (int j = 0; j <= NP; ++j) {
     if (j < pf || j > plst + 1) continue;
-    const double di = static_cast<double>(ib + j);
-    while (k < kmax - 1 && t2 <= di) {
+    const float dj = static_cast<float>(j);
+    while (k < kmax - 1 && t2 <= dj) {
       ++k;
-      k1 = k2;
       t1 = t2;
-      k2 = knot_at(kn, k + 1 - kmin);
-      t2 = pos(k2.x);
-      rw = __builtin_amdgcn_rcpf(static_cast<float>(t2 - t1));
+      r1 = r2;
+      q = knot_at(kn, k + 1 - kmin);
+      t2 = tau(q.x);
+      r2 = static_cast<float>(q.y - rbase);
+      rw = __builtin_amdgcn_rcpf(t2 - t1);
     }
     // fraction of bin k below the edge, clamped: 0 below the first knot,
     // 1 past the last
-    const float fr = __builtin_amdgcn_fmed3f(static_cast<float>(di - t1) * rw, 0.0f, 1.0f);
-    const double gv = fma(k2.y - k1.y, static_cast<double>(fr), k1.y);
-    if (j > pf) acc[j - 1] = fmaf(static_cast<float>(gv - gprev), scale, acc[j - 1]);
+    const float fr = __builtin_amdgcn_fmed3f((dj - t1) * rw, 0.0f, 1.0f);
+    const float gv = fmaf(r2 - r1, fr, r1);
+    if (j > pf) acc[j - 1] = fmaf(gv - gprev, scale, acc[j - 1]);
     gprev = gv;
   }
 }
